@@ -1,0 +1,15 @@
+"""fft_demo_amd -- MI355X (gfx950) matrix-core FFT and FFT-convolution.
+
+The compute path is the native library lib/libmfft.so (hand-written HIP kernels and a
+C++ planner behind the C ABI in include/mfft.h). Python modules here are thin handles:
+  plan     -- Plan (1D/2D batched FFT) over mfft_plan_create_* / mfft_exec
+  tcfft    -- the reference's tcfftCreate/tcfftExec/setup/doit/finalize interface
+"""
+from ._lib import (FORWARD, INVERSE, SCALE_NONE, SCALE_STAGE, ORDER_NATURAL, ORDER_DIGIT_REVERSED,
+                   PREC_FP16, PREC_BF16, PREC_SPLIT_FP16, PREC_FP32, FMT_C16, FMT_CBF16, FMT_C32,
+                   MfftError, load)
+from .plan import Plan, make_desc
+
+__all__ = ["Plan", "make_desc", "load", "MfftError", "FORWARD", "INVERSE", "SCALE_NONE", "SCALE_STAGE",
+           "ORDER_NATURAL", "ORDER_DIGIT_REVERSED", "PREC_FP16", "PREC_BF16", "PREC_SPLIT_FP16", "PREC_FP32",
+           "FMT_C16", "FMT_CBF16", "FMT_C32"]

@@ -1,0 +1,140 @@
+"""ctypes binding of the native library fft_demo_amd/lib/libmfft.so (C ABI in include/mfft.h).
+
+The native library is the product; this module only loads it and declares signatures.
+It raises if the library is missing: there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmfft.so")
+
+MFFT_OK = 0
+ERRORS = {-1: "MFFT_ERR_INVALID", -2: "MFFT_ERR_UNSUPPORTED", -3: "MFFT_ERR_HIP", -4: "MFFT_ERR_NOMEM"}
+
+PREC_FP16, PREC_BF16, PREC_SPLIT_FP16, PREC_FP32 = 0, 1, 2, 3
+FMT_C16, FMT_CBF16, FMT_C32 = 0, 1, 2
+FORWARD, INVERSE = -1, 1
+SCALE_NONE, SCALE_STAGE = 0, 1
+ORDER_NATURAL, ORDER_DIGIT_REVERSED = 0, 1
+
+PRECISIONS = {"fp16": PREC_FP16, "bf16": PREC_BF16, "split_fp16": PREC_SPLIT_FP16, "fp32": PREC_FP32}
+FORMATS = {"c16": FMT_C16, "cbf16": FMT_CBF16, "c32": FMT_C32}
+
+
+class MfftError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("scale", ctypes.c_int),
+        ("in_fmt", ctypes.c_int),
+        ("out_fmt", ctypes.c_int),
+        ("in_order", ctypes.c_int),
+        ("workspace_limit", ctypes.c_int64),
+        ("split_hint", ctypes.c_int),
+    ]
+
+
+class PassInfo(ctypes.Structure):
+    _fields_ = [
+        ("R", ctypes.c_int),
+        ("lines_per_tile", ctypes.c_int),
+        ("lane_lines", ctypes.c_int),
+        ("nstages", ctypes.c_int),
+        ("radix", ctypes.c_int * 8),
+        ("tiles", ctypes.c_int64),
+        ("generic", ctypes.c_int),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("batch", ctypes.c_int64),
+        ("npasses", ctypes.c_int),
+        ("passes", PassInfo * 6),
+        ("out_scale", ctypes.c_double),
+        ("workspace_bytes", ctypes.c_int64),
+        ("batch_chunk", ctypes.c_int64),
+        ("launches_per_exec", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "kind": self.kind,
+            "n": self.n,
+            "nx": self.nx,
+            "ny": self.ny,
+            "batch": self.batch,
+            "out_scale": self.out_scale,
+            "workspace_bytes": self.workspace_bytes,
+            "batch_chunk": self.batch_chunk,
+            "launches_per_exec": self.launches_per_exec,
+            "passes": [
+                {
+                    "R": p.R,
+                    "lines_per_tile": p.lines_per_tile,
+                    "lane_lines": p.lane_lines,
+                    "radix": list(p.radix[: p.nstages]),
+                    "tiles": p.tiles,
+                    "generic": bool(p.generic),
+                }
+                for p in self.passes[: self.npasses]
+            ],
+        }
+
+
+_lib = None
+
+# every symbol include/*.h declares, with its ctypes signature (restype, argtypes)
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64 = ctypes.c_int64
+SIGNATURES = {
+    # include/mfft.h
+    "mfft_desc_default": (ctypes.c_int, [ctypes.POINTER(Desc)]),
+    "mfft_plan_create_1d": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _i64, ctypes.POINTER(Desc)]),
+    "mfft_plan_create_2d": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _i64, ctypes.POINTER(Desc)]),
+    "mfft_plan_destroy": (ctypes.c_int, [_vp]),
+    "mfft_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(Info)]),
+    "mfft_exec": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "mfft_input_permutation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int]),
+    "mfft_last_error": (ctypes.c_char_p, []),
+    "mfft_version": (ctypes.c_char_p, []),
+}
+
+
+def load(path: str | None = None):
+    """Load libmfft.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(
+            f"native library {p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = ctypes.CDLL(p, mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int, where: str) -> None:
+    if code != MFFT_OK:
+        msg = load().mfft_last_error().decode(errors="replace")
+        raise MfftError(code, where, msg)

@@ -1,0 +1,71 @@
+// Shared definitions of the MI355X (gfx950) matrix-core FFT library: data formats,
+// precision modes and the argument block of the tile kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfft {
+
+// Complex element formats at a kernel boundary (interleaved re, im).
+enum Fmt : int {
+    FMT_C16 = 0,   // 2 x fp16  (4 B)  -- the reference's half2 layout (tcfft_half.h:20)
+    FMT_CBF16 = 1, // 2 x bf16  (4 B)
+    FMT_C32 = 2,   // 2 x fp32  (8 B)
+};
+
+// Arithmetic of the radix-16 stages.
+enum Prec : int {
+    PREC_FP16 = 0,  // fp16 MFMA operands, fp32 accumulate
+    PREC_BF16 = 1,  // bf16 MFMA operands, fp32 accumulate
+    PREC_SPLIT = 2, // split fp16 hi+lo operands (3 products), fp32 accumulate
+    PREC_FP32 = 3,  // plain fp32 VALU butterflies (the comparator)
+};
+
+constexpr int kMaxStages = 8;
+constexpr int kWaves = 4;                 // waves per workgroup of the tile kernel
+constexpr int kThreads = kWaves * 64;
+constexpr int kMaxTile = 4096;            // complex points per workgroup tile
+constexpr int kMaxPairsPerWave = kMaxTile / 256 / kWaves;  // MFMA column groups per wave/stage
+constexpr int kMaxValuPerThread = 32;     // complex values a thread holds in a VALU stage
+
+__host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int tile_lds_bytes(int points) { return lds_pad(points) * 8 + 64; }
+
+// One workgroup transforms T lines of R points. Lines are indexed by (l0, l1, l2);
+// l0 runs fastest and a tile covers T consecutive l0. Global element (line, point)
+// lives at  l0*s0 + l1*s1 + l2*s2 + point*ps  (in complex elements) on each side.
+struct TileArgs {
+    const void *in;
+    void *out;
+    int in_fmt, out_fmt;
+    int R;                     // points per line
+    int nstages;
+    int radix[kMaxStages];     // stage radices, product = R (16 -> matrix-core stage)
+    int T;                     // lines per tile (divides c0)
+    int TL;                    // lines interleaved in one MFMA column group (1..16)
+    int c0, c1, c2;            // line counts per level
+    long long in_s0, in_s1, in_s2, in_ps;
+    long long out_s0, out_s1, out_s2, out_ps;
+    const int *in_perm;        // optional: global point i holds logical point in_perm[i]
+    const float2 *stage_tw;    // W_R^e, e in [0, R), sign of the transform
+    const void *dft_ops;       // per-lane DFT16 operands (see DftOps)
+    // four-step twiddle after the last stage: out *= W_Ntw^(l0 * k), e = l0*k < Ntw
+    int tw_N;                  // 0 = no pass twiddle
+    int tw_level;              // 0: exponent l0 * k ; 1: exponent l1 * k
+    int tw_bits;               // W_N^e = tw_hi[e >> bits] * tw_lo[e & ((1<<bits)-1)]
+    const float2 *tw_lo, *tw_hi;
+    float stage_mul[kMaxStages];  // multiplier folded into each stage (1 or 1/radix)
+    float out_mul;             // extra multiplier applied at the store
+    int sign;                  // -1 forward, +1 inverse (VALU stages)
+    int in_vec, out_vec;       // base pointers are 16-byte aligned (vector copies allowed)
+};
+
+// DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of
+// [Fr | -s*Fi] (real output rows) and [s*Fi | Fr] (imag output rows), scaled by 16.
+struct alignas(16) DftOps {
+    _Float16 hr[64][8], hi_[64][8], lr[64][8], li[64][8];  // fp16 hi/lo, real and imag rows
+    __bf16 br[64][8], bi[64][8];                          // bf16 real and imag rows
+};
+
+}  // namespace mfft

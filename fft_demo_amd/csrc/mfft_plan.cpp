@@ -1,0 +1,592 @@
+// Host side of the matrix-core FFT: planner, device tables, exec and the C ABI.
+//
+// The reference plans a fixed radix/"merging" table per size (tcfftCreate,
+// tcfft_half.cu:1235-1345; tcfftCreate_2d, tcfft_half_2d.cu:1151-1221) and launches one
+// in-block pass plus strided merge passes (tcfftExec, tcfft_half.cu:1177-1233). This
+// planner decomposes any supported axis length with the four-step method instead:
+//   L = L1 * L2 :  pass A = L2 strided DFT_L1 (stride L2) times the twiddle W_L^(n2*k1),
+//                  then DFT_L2 on each row k1, written to k1 + L1*k2 (recursively).
+// Every pass is one launch of the tile kernel; intermediates live in a per-plan
+// workspace, processed in batch chunks small enough to stay in the 256 MB Infinity Cache.
+#include "../../include/mfft.h"
+#include "mfft_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace mfft {
+hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s);
+hipError_t gen_dft_ops(DftOps *ops, int sign, hipStream_t s);
+typedef void (*tile_fn)(TileArgs);
+tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
+int pow2_tile_lines(int R, int TL);
+hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream);
+}  // namespace mfft
+
+using namespace mfft;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(MFFT_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ factorisation
+static bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Stage radices of one in-tile line length, tail stage first (matches fft_tile_pow2).
+static bool line_radices(int R, std::vector<int> &out)
+{
+    out.clear();
+    if (R < 2) return false;
+    int n = R, e2 = 0;
+    while (n % 2 == 0) { n /= 2; ++e2; }
+    int tail2 = 1 << (e2 % 4);
+    int n16 = e2 / 4;
+    if (tail2 > 1) out.push_back(tail2);
+    // odd factors
+    const int odd[] = {17, 5, 3};
+    for (int p : odd)
+        while (n % p == 0) { out.push_back(p); n /= p; }
+    if (n != 1) return false;
+    for (int i = 0; i < n16; ++i) out.push_back(16);
+    return true;
+}
+
+// The reference-style radix list used for the digit-reversed input order:
+// radix-16 digits first, then the 2/4/8 tail (tcfft_half.cu:1250-1330 for 512, 1024).
+static void rev_radices(int n, std::vector<int> &out)
+{
+    out.clear();
+    int m = n, e2 = 0;
+    while (m % 2 == 0) { m /= 2; ++e2; }
+    for (int i = 0; i < e2 / 4; ++i) out.push_back(16);
+    if (e2 % 4) out.push_back(1 << (e2 % 4));
+    const int odd[] = {17, 5, 3};
+    for (int p : odd)
+        while (m % p == 0) { out.push_back(p); m /= p; }
+}
+
+static void gen_rev(int n, const std::vector<int> &radices, std::vector<int> &rev)
+{
+    // restates gen_rev (tcfft_half.cu:1351-1374)
+    std::vector<int> a(n), b(n);
+    for (int i = 0; i < n; ++i) a[i] = i;
+    int now = n;
+    for (int i = (int)radices.size() - 1; i >= 0; --i) {
+        int r = radices[i], sub = now / r;
+        for (int j = 0; j < n; j += now)
+            for (int k = 0; k < r; ++k)
+                for (int l = 0; l < sub; ++l) b[j + l + k * sub] = a[j + l * r + k];
+        now = sub;
+        std::swap(a, b);
+    }
+    rev = a;
+}
+
+// ------------------------------------------------------------------ plan structures
+struct Level {
+    long long count, in_stride, out_stride;
+    bool batch;  // the batch level (chunked at exec time)
+};
+
+enum Buf { BUF_IN = 0, BUF_OUT = 1, BUF_WORK = 2 };
+
+struct PassSpec {
+    int R;
+    long long ps_in, ps_out;
+    std::vector<Level> lines;
+    int tw_N = 0;
+    int tw_level = -1;  // index into lines
+    int src = BUF_IN, dst = BUF_OUT;
+    int perm_axis = -1; // axis whose digit-reversal is fused into this pass's loads
+    // derived
+    std::vector<int> radix;
+    bool generic = false;
+    int TL = 1, T = 1;
+    int l0 = 0, l1 = -1, l2 = -1;  // indices into lines
+    const void *fn = nullptr;
+    float2 *tw_lo = nullptr, *tw_hi = nullptr;
+    int tw_bits = 0;
+    float2 *stage_tw = nullptr;
+    int *perm = nullptr;
+};
+
+struct mfft_plan_s {
+    int kind = 1;
+    int n = 0, nx = 0, ny = 0;
+    long long batch = 0;
+    mfft_desc d{};
+    long long total = 0;      // complex elements per transform
+    int mid_fmt = FMT_C32;
+    std::vector<PassSpec> passes;
+    DftOps *ops = nullptr;
+    void *work = nullptr;
+    long long work_bytes = 0;
+    long long chunk = 1;
+    double out_scale = 1.0;
+    std::vector<void *> allocs;
+    std::vector<int> rev_axis[2];
+    std::mutex exec_mu;       // the workspace is per plan
+};
+
+static int fmt_bytes(int fmt) { return fmt == FMT_C32 ? 8 : 4; }
+
+// ------------------------------------------------------------------ axis planning
+static bool single_pass_ok(int R, long long ps_in, const std::vector<Level> &lines)
+{
+    if (R < 2 || R > kMaxTile) return false;
+    std::vector<int> rad;
+    if (!line_radices(R, rad)) return false;
+    if (ps_in == 1) return true;
+    bool has_unit = false;
+    for (auto &l : lines)
+        if (l.in_stride == 1) has_unit = true;
+    return has_unit ? R * 16 <= kMaxTile : true;
+}
+
+static bool factorable(long long n)
+{
+    std::vector<int> r;
+    return n <= kMaxTile && line_radices((int)n, r);
+}
+
+// Append the passes of a length-L DFT along an axis. Input element (line, j) is at
+// line_in + j*ps_in, output element (line, k) at line_out + k*ps_out.
+static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, long long ps_out,
+                     std::vector<Level> lines, int src, int dst, int perm_axis, int split_hint)
+{
+    if (single_pass_ok((int)std::min<long long>(L, 1 << 30), ps_in, lines)) {
+        PassSpec p;
+        p.R = (int)L;
+        p.ps_in = ps_in;
+        p.ps_out = ps_out;
+        p.lines = lines;
+        p.src = src;
+        p.dst = dst;
+        p.perm_axis = perm_axis;
+        out.push_back(p);
+        return MFFT_OK;
+    }
+    if (dst == BUF_WORK)
+        return fail(MFFT_ERR_UNSUPPORTED, "an inner axis longer than %d points is not supported", kMaxTile);
+    if (perm_axis >= 0)
+        return fail(MFFT_ERR_UNSUPPORTED,
+                    "digit-reversed input is supported for axes done in one pass (length <= %d)", kMaxTile);
+    // choose L1 (strided pass, <= 256) and L2 = L / L1
+    long long L1 = 0;
+    if (split_hint > 1 && L % split_hint == 0 && split_hint <= 256 && factorable(split_hint)) {
+        L1 = split_hint;
+    } else if (is_pow2(L)) {
+        int m = 0;
+        while ((1LL << m) < L) ++m;
+        int m1 = (m + 1) / 2;
+        if (m1 > 8) m1 = 8;
+        if (m - m1 > 12 && m1 < 8) m1 = std::min(8, m - 12);
+        L1 = 1LL << m1;
+    } else {
+        for (long long c = 256; c >= 2; --c)
+            if (L % c == 0 && factorable(c) && (L / c <= kMaxTile ? factorable(L / c) : true)) {
+                L1 = c;
+                break;
+            }
+    }
+    if (L1 < 2 || L % L1 != 0) return fail(MFFT_ERR_UNSUPPORTED, "cannot factor axis length %lld", L);
+    long long L2 = L / L1;
+    // pass A: DFT_L1 over n1 (stride L2*ps_in) for each (line, n2); in-place layout
+    PassSpec a;
+    a.R = (int)L1;
+    a.ps_in = L2 * ps_in;
+    a.ps_out = L2 * ps_in;
+    a.lines = lines;
+    for (auto &l : a.lines) l.out_stride = l.in_stride;  // pass A keeps the input layout
+    a.lines.push_back(Level{L2, ps_in, ps_in, false});
+    a.tw_N = (int)L;
+    a.tw_level = (int)a.lines.size() - 1;
+    a.src = src;
+    a.dst = BUF_WORK;
+    out.push_back(a);
+    // remaining: DFT_L2 over n2 for each (line, k1); output at k1 + L1*k2
+    // input side of the sub-problem reads the workspace laid out like the input
+    std::vector<Level> sub2;
+    for (auto &l : lines) sub2.push_back(Level{l.count, l.in_stride, l.out_stride, l.batch});
+    sub2.push_back(Level{L1, L2 * ps_in, ps_out, false});
+    return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, BUF_WORK, dst, -1, 0);
+}
+
+// Choose tile geometry and kernel for every pass.
+static int finish_passes(mfft_plan_s *p)
+{
+    for (auto &ps : p->passes) {
+        if (!line_radices(ps.R, ps.radix)) return fail(MFFT_ERR_UNSUPPORTED, "line length %d", ps.R);
+        // merge adjacent levels that are contiguous on both sides (never the batch level)
+        // choose l0
+        int unit_in = -1, unit_out = -1;
+        for (int i = 0; i < (int)ps.lines.size(); ++i) {
+            if (ps.lines[i].in_stride == 1 && unit_in < 0) unit_in = i;
+            if (ps.lines[i].out_stride == 1 && unit_out < 0) unit_out = i;
+        }
+        bool strided = ps.ps_in != 1;
+        bool pow2 = is_pow2(ps.R) && ps.R >= 16;
+        if (strided && unit_in >= 0 && ps.R * 16 <= kMaxTile) {
+            ps.l0 = unit_in;
+            ps.TL = 16;
+        } else if (!strided) {
+            ps.TL = 1;
+            if (ps.ps_out != 1 && unit_out >= 0)
+                ps.l0 = unit_out;
+            else {
+                // the level with the smallest input stride
+                int best = 0;
+                for (int i = 1; i < (int)ps.lines.size(); ++i)
+                    if (ps.lines[i].in_stride < ps.lines[best].in_stride) best = i;
+                ps.l0 = best;
+            }
+        } else {
+            ps.l0 = 0;
+            ps.TL = 1;
+            pow2 = false;
+        }
+        ps.generic = !pow2;
+        if (!ps.generic) {
+            ps.T = pow2_tile_lines(ps.R, ps.TL);
+        } else {
+            int t = kMaxTile / ps.R;
+            if (ps.TL == 16 && t < 16) ps.TL = 1;
+            ps.T = std::max(1, t);
+            if (ps.TL > ps.T) ps.TL = 1;
+            if (ps.T % ps.TL) ps.T = ps.TL;
+        }
+        // remaining levels -> l1, l2 (at most two besides l0)
+        std::vector<int> rest;
+        if (ps.tw_N && ps.tw_level != ps.l0) rest.push_back(ps.tw_level);
+        for (int i = 0; i < (int)ps.lines.size(); ++i)
+            if (i != ps.l0 && !(ps.tw_N && i == ps.tw_level)) rest.push_back(i);
+        if (rest.size() > 2) return fail(MFFT_ERR_UNSUPPORTED, "too many line levels");
+        ps.l1 = rest.size() > 0 ? rest[0] : -1;
+        ps.l2 = rest.size() > 1 ? rest[1] : -1;
+        if (ps.tw_N && ps.tw_level != ps.l0 && ps.tw_level != ps.l1)
+            return fail(MFFT_ERR_UNSUPPORTED, "twiddle level layout");
+        ps.fn = (const void *)find_tile_kernel(p->d.precision, ps.R, ps.TL, ps.generic);
+        if (!ps.fn) return fail(MFFT_ERR_UNSUPPORTED, "no kernel for R=%d TL=%d", ps.R, ps.TL);
+    }
+    return MFFT_OK;
+}
+
+static int alloc_dev(mfft_plan_s *p, void **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(MFFT_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    p->allocs.push_back(*ptr);
+    return MFFT_OK;
+}
+
+static int build_tables(mfft_plan_s *p)
+{
+    int sign = p->d.direction;
+    if (alloc_dev(p, (void **)&p->ops, sizeof(DftOps))) return MFFT_ERR_NOMEM;
+    HIP_TRY(gen_dft_ops(p->ops, sign, 0));
+    for (auto &ps : p->passes) {
+        if (alloc_dev(p, (void **)&ps.stage_tw, sizeof(float2) * ps.R)) return MFFT_ERR_NOMEM;
+        HIP_TRY(gen_twiddle(ps.stage_tw, ps.R, ps.R, 1, sign, 0));
+        if (ps.tw_N) {
+            int bits = 0;
+            while ((1LL << (2 * bits)) < ps.tw_N) ++bits;
+            long long nlo = 1LL << bits, nhi = (ps.tw_N + nlo - 1) / nlo;
+            ps.tw_bits = bits;
+            if (alloc_dev(p, (void **)&ps.tw_lo, sizeof(float2) * nlo)) return MFFT_ERR_NOMEM;
+            if (alloc_dev(p, (void **)&ps.tw_hi, sizeof(float2) * nhi)) return MFFT_ERR_NOMEM;
+            HIP_TRY(gen_twiddle(ps.tw_lo, ps.tw_N, nlo, 1, sign, 0));
+            HIP_TRY(gen_twiddle(ps.tw_hi, ps.tw_N, nhi, nlo, sign, 0));
+        }
+        if (ps.perm_axis >= 0 && p->d.in_order == MFFT_ORDER_DIGIT_REVERSED) {
+            const std::vector<int> &rev = p->rev_axis[ps.perm_axis];
+            if ((int)rev.size() != ps.R) return fail(MFFT_ERR_INVALID, "permutation size");
+            if (alloc_dev(p, (void **)&ps.perm, sizeof(int) * ps.R)) return MFFT_ERR_NOMEM;
+            HIP_TRY(hipMemcpy(ps.perm, rev.data(), sizeof(int) * ps.R, hipMemcpyHostToDevice));
+        }
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return MFFT_OK;
+}
+
+static int finalize_plan(mfft_plan_s *p)
+{
+    int rc = finish_passes(p);
+    if (rc) return rc;
+    // scale bookkeeping
+    double sc = 1.0;
+    for (auto &ps : p->passes)
+        for (int r : ps.radix) sc /= (p->d.scale == MFFT_SCALE_STAGE ? r : 1);
+    p->out_scale = sc;
+    // workspace and batch chunking
+    p->mid_fmt = (p->d.precision == MFFT_PREC_FP16) ? FMT_C16
+                 : (p->d.precision == MFFT_PREC_BF16) ? FMT_CBF16
+                                                      : FMT_C32;
+    bool need_work = false;
+    for (auto &ps : p->passes)
+        if (ps.src == BUF_WORK || ps.dst == BUF_WORK) need_work = true;
+    long long per = p->total * fmt_bytes(p->mid_fmt);
+    long long limit = p->d.workspace_limit > 0 ? p->d.workspace_limit : (64LL << 20);
+    p->chunk = p->batch;
+    if (need_work) {
+        long long c = std::max(1LL, limit / std::max(1LL, per));
+        p->chunk = std::min(p->batch, c);
+        p->work_bytes = p->chunk * per;
+        if (alloc_dev(p, &p->work, (size_t)p->work_bytes)) return MFFT_ERR_NOMEM;
+    }
+    return build_tables(p);
+}
+
+static int check_desc(const mfft_desc *d)
+{
+    if (d->precision < 0 || d->precision > 3) return fail(MFFT_ERR_INVALID, "precision %d", d->precision);
+    if (d->direction != -1 && d->direction != 1) return fail(MFFT_ERR_INVALID, "direction %d", d->direction);
+    if (d->scale != 0 && d->scale != 1) return fail(MFFT_ERR_INVALID, "scale %d", d->scale);
+    if (d->in_fmt < 0 || d->in_fmt > 2 || d->out_fmt < 0 || d->out_fmt > 2)
+        return fail(MFFT_ERR_INVALID, "format");
+    if (d->in_order != 0 && d->in_order != 1) return fail(MFFT_ERR_INVALID, "in_order");
+    return MFFT_OK;
+}
+
+// ------------------------------------------------------------------ exec
+static void *buf_ptr(int which, const void *in, void *out, void *work)
+{
+    return which == BUF_IN ? const_cast<void *>(in) : which == BUF_OUT ? out : work;
+}
+
+static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s)
+{
+    for (auto &ps : p->passes) {
+        TileArgs a;
+        memset(&a, 0, sizeof a);
+        a.in = buf_ptr(ps.src, in, out, p->work);
+        a.out = buf_ptr(ps.dst, in, out, p->work);
+        a.in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
+        a.out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+        a.R = ps.R;
+        a.nstages = (int)ps.radix.size();
+        for (int i = 0; i < a.nstages; ++i) {
+            a.radix[i] = ps.radix[i];
+            a.stage_mul[i] = p->d.scale == MFFT_SCALE_STAGE ? 1.0f / ps.radix[i] : 1.0f;
+        }
+        a.T = ps.T;
+        a.TL = ps.TL;
+        auto cnt = [&](int li) -> long long {
+            if (li < 0) return 1;
+            return ps.lines[li].batch ? nb : ps.lines[li].count;
+        };
+        a.c0 = (int)cnt(ps.l0);
+        a.c1 = (int)cnt(ps.l1);
+        a.c2 = (int)cnt(ps.l2);
+        a.in_s0 = ps.lines[ps.l0].in_stride;
+        a.out_s0 = ps.lines[ps.l0].out_stride;
+        a.in_s1 = ps.l1 >= 0 ? ps.lines[ps.l1].in_stride : 0;
+        a.out_s1 = ps.l1 >= 0 ? ps.lines[ps.l1].out_stride : 0;
+        a.in_s2 = ps.l2 >= 0 ? ps.lines[ps.l2].in_stride : 0;
+        a.out_s2 = ps.l2 >= 0 ? ps.lines[ps.l2].out_stride : 0;
+        a.in_ps = ps.ps_in;
+        a.out_ps = ps.ps_out;
+        a.in_perm = ps.perm;
+        a.stage_tw = ps.stage_tw;
+        a.dft_ops = p->ops;
+        a.tw_N = ps.tw_N;
+        a.tw_level = ps.tw_level == ps.l0 ? 0 : 1;
+        a.tw_bits = ps.tw_bits;
+        a.tw_lo = ps.tw_lo;
+        a.tw_hi = ps.tw_hi;
+        a.out_mul = 1.0f;
+        a.sign = p->d.direction;
+        a.in_vec = ((uintptr_t)a.in & 15) == 0;
+        a.out_vec = ((uintptr_t)a.out & 15) == 0;
+        long long ntiles = ((a.c0 + a.T - 1) / a.T) * (long long)a.c1 * a.c2;
+        if (ntiles <= 0) continue;
+        if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
+        HIP_TRY(launch_tile(ps.fn, a, ntiles, s));
+    }
+    return MFFT_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int mfft_desc_default(mfft_desc *d)
+{
+    if (!d) return fail(MFFT_ERR_INVALID, "null desc");
+    memset(d, 0, sizeof *d);
+    d->precision = MFFT_PREC_SPLIT_FP16;
+    d->direction = MFFT_FORWARD;
+    d->scale = MFFT_SCALE_STAGE;
+    d->in_fmt = MFFT_FMT_C16;
+    d->out_fmt = MFFT_FMT_C16;
+    d->in_order = MFFT_ORDER_NATURAL;
+    return MFFT_OK;
+}
+
+int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d)
+{
+    if (!plan || !d) return fail(MFFT_ERR_INVALID, "null argument");
+    *plan = nullptr;
+    if (n < 2 || batch < 1) return fail(MFFT_ERR_INVALID, "n=%d batch=%lld", n, (long long)batch);
+    int rc = check_desc(d);
+    if (rc) return rc;
+    mfft_plan_s *p = new mfft_plan_s;
+    p->kind = 1;
+    p->n = n;
+    p->batch = batch;
+    p->d = *d;
+    p->total = n;
+    if (d->in_order == MFFT_ORDER_DIGIT_REVERSED) {
+        std::vector<int> rad;
+        rev_radices(n, rad);
+        gen_rev(n, rad, p->rev_axis[0]);
+    }
+    std::vector<Level> lines{Level{batch, n, n, true}};
+    rc = plan_axis(p->passes, n, 1, 1, lines, BUF_IN, BUF_OUT,
+                   d->in_order == MFFT_ORDER_DIGIT_REVERSED ? 0 : -1, d->split_hint);
+    if (!rc) rc = finalize_plan(p);
+    if (rc) {
+        std::string msg = g_err;
+        mfft_plan_destroy(p);
+        g_err = msg;
+        return rc;
+    }
+    *plan = p;
+    return MFFT_OK;
+}
+
+int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d)
+{
+    if (!plan || !d) return fail(MFFT_ERR_INVALID, "null argument");
+    *plan = nullptr;
+    if (nx < 2 || ny < 2 || batch < 1) return fail(MFFT_ERR_INVALID, "nx=%d ny=%d batch=%lld", nx, ny, (long long)batch);
+    int rc = check_desc(d);
+    if (rc) return rc;
+    mfft_plan_s *p = new mfft_plan_s;
+    p->kind = 2;
+    p->nx = nx;
+    p->ny = ny;
+    p->n = nx * ny;
+    p->batch = batch;
+    p->d = *d;
+    p->total = (long long)nx * ny;
+    bool rev = d->in_order == MFFT_ORDER_DIGIT_REVERSED;
+    if (rev) {
+        std::vector<int> rad;
+        rev_radices(nx, rad);
+        gen_rev(nx, rad, p->rev_axis[0]);
+        rev_radices(ny, rad);
+        gen_rev(ny, rad, p->rev_axis[1]);
+    }
+    long long NN = (long long)nx * ny;
+    // rows (y axis, contiguous): lines = (x, b)
+    std::vector<Level> rows{Level{nx, ny, ny, false}, Level{batch, NN, NN, true}};
+    rc = plan_axis(p->passes, ny, 1, 1, rows, BUF_IN, BUF_WORK, rev ? 1 : -1, 0);
+    // columns (x axis, stride ny): lines = (y, b)
+    if (!rc) {
+        std::vector<Level> cols{Level{ny, 1, 1, false}, Level{batch, NN, NN, true}};
+        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, 0);
+    }
+    if (!rc) rc = finalize_plan(p);
+    if (rc) {
+        std::string msg = g_err;
+        mfft_plan_destroy(p);
+        g_err = msg;
+        return rc;
+    }
+    *plan = p;
+    return MFFT_OK;
+}
+
+int mfft_plan_destroy(mfft_plan plan)
+{
+    if (!plan) return MFFT_OK;
+    (void)hipDeviceSynchronize();
+    for (void *ptr : plan->allocs) (void)hipFree(ptr);
+    delete plan;
+    return MFFT_OK;
+}
+
+int mfft_plan_info(mfft_plan p, mfft_info *info)
+{
+    if (!p || !info) return fail(MFFT_ERR_INVALID, "null argument");
+    memset(info, 0, sizeof *info);
+    info->kind = p->kind;
+    info->n = p->n;
+    info->nx = p->nx;
+    info->ny = p->ny;
+    info->batch = p->batch;
+    info->npasses = (int)std::min<size_t>(p->passes.size(), MFFT_MAX_PASSES);
+    long long nchunks = (p->batch + p->chunk - 1) / p->chunk;
+    for (int i = 0; i < info->npasses; ++i) {
+        const PassSpec &ps = p->passes[i];
+        mfft_pass_info &pi = info->pass[i];
+        pi.R = ps.R;
+        pi.lines_per_tile = ps.T;
+        pi.lane_lines = ps.TL;
+        pi.nstages = (int)ps.radix.size();
+        for (int k = 0; k < pi.nstages && k < 8; ++k) pi.radix[k] = ps.radix[k];
+        long long lines = 1;
+        for (auto &l : ps.lines) lines *= l.batch ? p->chunk : l.count;
+        long long c0 = ps.lines[ps.l0].batch ? p->chunk : ps.lines[ps.l0].count;
+        pi.tiles = (lines / c0) * ((c0 + ps.T - 1) / ps.T);
+        pi.generic = ps.generic;
+    }
+    info->out_scale = p->out_scale;
+    info->workspace_bytes = p->work_bytes;
+    info->batch_chunk = p->chunk;
+    info->launches_per_exec = nchunks * (long long)p->passes.size();
+    return MFFT_OK;
+}
+
+int mfft_exec(mfft_plan p, const void *in, void *out, void *stream)
+{
+    if (!p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(p->exec_mu);
+    hipStream_t s = (hipStream_t)stream;
+    long long ib = p->total * fmt_bytes(p->d.in_fmt), ob = p->total * fmt_bytes(p->d.out_fmt);
+    for (long long b0 = 0; b0 < p->batch; b0 += p->chunk) {
+        long long nb = std::min(p->chunk, p->batch - b0);
+        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s);
+        if (rc) return rc;
+    }
+    return MFFT_OK;
+}
+
+int mfft_input_permutation(mfft_plan p, int axis, int *rev, int n)
+{
+    if (!p || !rev || axis < 0 || axis > 1) return fail(MFFT_ERR_INVALID, "bad argument");
+    int len = p->kind == 1 ? p->n : (axis == 0 ? p->nx : p->ny);
+    if (n != len) return fail(MFFT_ERR_INVALID, "length %d != %d", n, len);
+    std::vector<int> rad, r;
+    rev_radices(len, rad);
+    gen_rev(len, rad, r);
+    memcpy(rev, r.data(), sizeof(int) * len);
+    return MFFT_OK;
+}
+
+const char *mfft_last_error(void) { return g_err.c_str(); }
+
+const char *mfft_version(void) { return "mfft 0.1 (gfx950 matrix-core FFT)"; }
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+/*
+ * mfft -- MI355X (gfx950) matrix-core FFT and FFT-convolution library, C ABI.
+ *
+ * This is the native core under the reference's plan/exec API (tcfft_half.h,
+ * tcfft_half_2d.h, tensorcore_fft_conv2d.h; see those headers in this directory for the
+ * drop-in names). All entry points are extern "C", take plain pointers and sizes and
+ * return an int status (MFFT_OK = 0, negative on error; mfft_last_error() gives the
+ * message of the calling thread's last failure). Device pointers are HIP device memory;
+ * streams are hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Conventions (reference: main.cpp:68-94, tensorcore_fft_conv2d_kernel.cu:47-82):
+ *   - complex data is interleaved (re, im), element index fastest, then batch;
+ *   - MFFT_FORWARD computes sum_n x[n] exp(-2*pi*i*n*k/N), MFFT_INVERSE uses +i;
+ *   - 2D data is [batch][nx][ny] with y contiguous (tcfft_half_2d.cu:1264-1271);
+ *   - the output equals out_scale * DFT(in), where out_scale (mfft_plan_info) is 1 for
+ *     MFFT_SCALE_NONE (the reference's unnormalised contract) and 1/(product of the
+ *     transform lengths) for MFFT_SCALE_STAGE (every radix-r stage multiplies by 1/r,
+ *     so no intermediate can exceed the input's magnitude -- the fp16 overflow guard).
+ */
+#ifndef MFFT_H
+#define MFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MFFT_OK = 0,
+    MFFT_ERR_INVALID = -1,     /* bad argument */
+    MFFT_ERR_UNSUPPORTED = -2, /* size/format combination not provided */
+    MFFT_ERR_HIP = -3,         /* a HIP runtime call failed */
+    MFFT_ERR_NOMEM = -4,       /* device allocation failed or budget exceeded */
+};
+
+/* arithmetic of the radix-16 stages */
+enum {
+    MFFT_PREC_FP16 = 0,       /* fp16 matrix-core operands, fp32 accumulate */
+    MFFT_PREC_BF16 = 1,       /* bf16 matrix-core operands, fp32 accumulate */
+    MFFT_PREC_SPLIT_FP16 = 2, /* split fp16 hi+lo operands, 3 products, fp32 accumulate */
+    MFFT_PREC_FP32 = 3,       /* plain fp32 vector-ALU butterflies (comparator) */
+};
+
+/* complex element formats */
+enum {
+    MFFT_FMT_C16 = 0,   /* 2 x fp16 (the reference's half2) */
+    MFFT_FMT_CBF16 = 1, /* 2 x bf16 */
+    MFFT_FMT_C32 = 2,   /* 2 x fp32 */
+};
+
+enum { MFFT_FORWARD = -1, MFFT_INVERSE = 1 };
+enum { MFFT_SCALE_NONE = 0, MFFT_SCALE_STAGE = 1 };
+/* input order: natural, or the reference's mixed-radix digit-reversed order
+ * (tcfft_half.cu:1351-1374, applied by setup() at tcfft_half.cu:1385-1391) */
+enum { MFFT_ORDER_NATURAL = 0, MFFT_ORDER_DIGIT_REVERSED = 1 };
+
+typedef struct mfft_desc {
+    int precision;            /* MFFT_PREC_* */
+    int direction;            /* MFFT_FORWARD / MFFT_INVERSE */
+    int scale;                /* MFFT_SCALE_* */
+    int in_fmt, out_fmt;      /* MFFT_FMT_* */
+    int in_order;             /* MFFT_ORDER_* */
+    int64_t workspace_limit;  /* bytes of intermediate buffer per plan (0 = default) */
+    int split_hint;           /* first-pass length for multi-pass 1D (0 = planner/autotuner) */
+} mfft_desc;
+
+typedef struct mfft_plan_s *mfft_plan;
+
+#define MFFT_MAX_PASSES 6
+typedef struct mfft_pass_info {
+    int R;          /* points per line of the pass */
+    int lines_per_tile;
+    int lane_lines; /* lines interleaved in one matrix-core column group */
+    int nstages;
+    int radix[8];
+    int64_t tiles;  /* workgroups per launch (per batch chunk) */
+    int generic;    /* 1 = runtime-schedule kernel (odd radices) */
+} mfft_pass_info;
+
+typedef struct mfft_info {
+    int kind;              /* 1 = 1D, 2 = 2D */
+    int n, nx, ny;
+    int64_t batch;
+    int npasses;
+    mfft_pass_info pass[MFFT_MAX_PASSES];
+    double out_scale;      /* out = out_scale * DFT(in) */
+    int64_t workspace_bytes;
+    int64_t batch_chunk;   /* transforms per chunk (intermediates stay cache-resident) */
+    int64_t launches_per_exec;
+} mfft_info;
+
+/* Fill a descriptor with defaults: split fp16, forward, per-stage scaling, fp16 in/out,
+ * natural order. */
+int mfft_desc_default(mfft_desc *d);
+
+/* Plans own their device tables and workspace; they are reentrant (exec keeps no mutable
+ * state besides the workspace, so two streams need two plans). Replaces tcfftCreate
+ * (tcfft_half.cu:1235-1345) and tcfftCreate_2d (tcfft_half_2d.cu:1151-1221). */
+int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d);
+int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d);
+int mfft_plan_destroy(mfft_plan plan);
+int mfft_plan_info(mfft_plan plan, mfft_info *info);
+
+/* Execute on `stream`. in == out is allowed (in place, as tcfftExec). Replaces
+ * tcfftExec (tcfft_half.cu:1177-1233), tcfftExec_2d (tcfft_half_2d.cu:1128-1149) and
+ * TensorCoreFFTExec (tensorcore_fft_conv2d_kernel.cu:1495-1525). */
+int mfft_exec(mfft_plan plan, const void *in, void *out, void *stream);
+
+/* Digit-reversal table of the plan's radix list (gen_rev, tcfft_half.cu:1351-1374):
+ * position i of a digit-reversed buffer holds natural element rev[i]. For 2D plans,
+ * axis 0 = x (rev_x), axis 1 = y (rev_y). */
+int mfft_input_permutation(mfft_plan plan, int axis, int *rev, int n);
+
+/* Message of the calling thread's last error ("" if none). */
+const char *mfft_last_error(void);
+
+/* Library version string and the device's properties as seen by the library. */
+const char *mfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFFT_H */

@@ -1,0 +1,71 @@
+"""GPU parity of the batched 1D matrix-core FFT against the FP64 oracle.
+
+Tolerances (relative L2 / relative max vs FP64, input already rounded to the boundary
+format): split fp16 <= 1e-3, fp16 <= 4e-3, bf16 <= 3e-2,
+plain fp32 <= 1e-5. out_scale (1 or 1/N) is divided out before comparing.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import complex_input, from_device, rel_errors, to_device
+
+TOL = {"split_fp16": (1e-3, 4e-3), "fp16": (4e-3, 2e-2), "bf16": (3e-2, 1e-1), "fp32": (1e-5, 1e-4)}
+FMT = {"split_fp16": "c16", "fp16": "c16", "bf16": "cbf16", "fp32": "c32"}
+
+
+def _run(dev, n, batch, prec, direction=-1, seed=11, scale=None, **kw):
+    import fft_demo_amd as F
+
+    plan = F.Plan((n,), batch, precision=prec, direction=direction, scale=scale, **kw)
+    x = complex_input((batch, n), seed, FMT[prec] if "in_fmt" not in kw else kw["in_fmt"])
+    xd = to_device(x, plan.in_dtype(), dev)
+    y = from_device(plan.exec(xd)) / plan.out_scale
+    ref = oracle.fft(x, sign=direction)
+    return y, ref, plan
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp16", "bf16", "fp32"])
+def test_n256_batch4096(dev, prec):
+    """The reference's only working plan (tcfft_half.cu:1252-1255), N=256 x 4096."""
+    y, ref, plan = _run(dev, 256, 4096, prec)
+    l2, mx = rel_errors(y, ref)
+    print(f"N=256 b=4096 {prec}: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 512, 1024, 2048, 4096, 8192, 65536])
+def test_sizes_split(dev, n):
+    batch = max(1, min(64, (1 << 20) // n))
+    y, ref, plan = _run(dev, n, batch, "split_fp16")
+    l2, mx = rel_errors(y, ref)
+    print(f"N={n} b={batch} split: rel_l2={l2:.3e} rel_max={mx:.3e} passes={len(plan.info['passes'])}")
+    assert l2 <= 1e-3 and mx <= 4e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [256, 4096, 65536])
+def test_inverse_split(dev, n):
+    y, ref, plan = _run(dev, n, 4, "split_fp16", direction=1)
+    l2, mx = rel_errors(y, ref)
+    assert l2 <= 1e-3 and mx <= 4e-3
+
+
+@pytest.mark.gpu
+def test_odd_batch_partial_tiles(dev):
+    """Batch counts that leave a partial last tile."""
+    for n, b in [(256, 37), (1024, 5), (64, 3)]:
+        y, ref, _ = _run(dev, n, b, "split_fp16", seed=5)
+        l2, _ = rel_errors(y, ref)
+        assert l2 <= 1e-3, (n, b, l2)
+
+
+@pytest.mark.gpu
+def test_fp32_io_split(dev):
+    """Split precision with fp32 input and output (no output quantisation)."""
+    y, ref, _ = _run(dev, 4096, 8, "split_fp16", in_fmt="c32", out_fmt="c32")
+    l2, mx = rel_errors(y, ref)
+    print(f"split c32 io: {l2:.3e} {mx:.3e}")
+    assert l2 <= 1e-4
