@@ -52,6 +52,9 @@ class PassInfo(ctypes.Structure):
         ("radix", ctypes.c_int * 8),
         ("tiles", ctypes.c_int64),
         ("generic", ctypes.c_int),
+        ("in_elem_bytes", ctypes.c_int),
+        ("out_elem_bytes", ctypes.c_int),
+        ("points", ctypes.c_int64),
     ]
 
 
@@ -89,6 +92,9 @@ class Info(ctypes.Structure):
                     "radix": list(p.radix[: p.nstages]),
                     "tiles": p.tiles,
                     "generic": bool(p.generic),
+                    "in_elem_bytes": p.in_elem_bytes,
+                    "out_elem_bytes": p.out_elem_bytes,
+                    "points": p.points,
                 }
                 for p in self.passes[: self.npasses]
             ],
@@ -109,6 +115,7 @@ SIGNATURES = {
     "mfft_plan_destroy": (ctypes.c_int, [_vp]),
     "mfft_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(Info)]),
     "mfft_exec": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "mfft_exec_events": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int]),
     "mfft_input_permutation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int]),
     "mfft_last_error": (ctypes.c_char_p, []),
     "mfft_version": (ctypes.c_char_p, []),

@@ -27,7 +27,8 @@ constexpr int kWaves = 4;                 // waves per workgroup of the tile ker
 constexpr int kThreads = kWaves * 64;
 constexpr int kMaxTile = 4096;            // complex points per workgroup tile
 constexpr int kMaxPairsPerWave = kMaxTile / 256 / kWaves;  // MFMA column groups per wave/stage
-constexpr int kMaxValuPerThread = 32;     // complex values a thread holds in a VALU stage
+constexpr int kMaxValuPerThread = 32;
+constexpr int kFastTw = 256;              // four-step table entries staged in LDS (each half)     // complex values a thread holds in a VALU stage
 
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
 __host__ __device__ constexpr int tile_lds_bytes(int points) { return lds_pad(points) * 8 + 64; }
@@ -59,6 +60,12 @@ struct TileArgs {
     float out_mul;             // extra multiplier applied at the store
     int sign;                  // -1 forward, +1 inverse (VALU stages)
     int in_vec, out_vec;       // base pointers are 16-byte aligned (vector copies allowed)
+    // fast kernels: scale folded into the tables
+    const float2 *stage_tw_s;  // W_R^e * (per-stage scale, / 16 for matrix-core operands)
+    const float2 *tw_lo_s;     // tw_lo * last-stage scale
+    float last_mul;            // last-stage scale when there is no pass twiddle
+    float tail_mul;            // scale of the radix-2/4/8 stage
+    int tw_lds;                // four-step tables fit in LDS (2^tw_bits and the hi part <= kFastTw)
 };
 
 // DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of

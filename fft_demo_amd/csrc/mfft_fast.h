@@ -1,0 +1,28 @@
+// Fast tile kernels of the matrix-core FFT (power-of-two line lengths >= 256).
+//
+// Same method as the generic tile kernel (radix-16 DFT-matrix products on MFMA, Stockham
+// schedule, LDS between stages) with the HBM traffic moved into the stages themselves:
+// the first stage loads its MFMA operands straight from global memory, the last stage
+// stores its accumulators straight to global memory (with the four-step twiddle and the
+// output conversion fused), and LDS is touched only between stages. Geometry, formats
+// and stage count are compile-time, so index arithmetic folds to shifts.
+//
+// Kinds (how a tile of T lines x R points maps onto memory):
+//   KIND_ROWS     points contiguous (in_ps = out_ps = 1); lanes walk the points of a line.
+//   KIND_ROWS_TR  points contiguous on input, lines contiguous on output (out_s0 = 1):
+//                 the last stage switches the lanes to walk the lines.
+//   KIND_COLS     lines contiguous on both sides (in_s0 = out_s0 = 1, T = 16); lanes walk
+//                 the 16 lines of the tile (strided passes of the four-step method).
+#pragma once
+#include "mfft_common.h"
+
+namespace mfft {
+
+enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2 };
+
+typedef void (*fast_fn)(TileArgs);
+fast_fn find_fast_kernel(int prec, int R, int kind, int in_fmt, int out_fmt);
+int fast_tile_lines(int R, int kind);
+int fast_threads(int prec);
+
+}  // namespace mfft

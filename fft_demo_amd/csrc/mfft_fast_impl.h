@@ -1,0 +1,600 @@
+// Implementation of the fast tile kernels (included by one .hip file per precision).
+#pragma once
+#include "mfft_fast.h"
+
+namespace mfft {
+namespace fast {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+enum { MAP_POINTS = 0, MAP_LINES = 1 };
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ------------------------------------------------------------------ element formats
+template <int FMT>
+struct Elem;
+template <>
+struct Elem<FMT_C16> {
+    typedef uint32_t raw;
+    static __device__ __forceinline__ float2 f(raw w)
+    {
+        _Float16 re = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+        _Float16 im = __builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+        return make_float2((float)re, (float)im);
+    }
+    static __device__ __forceinline__ raw pack(float2 v)
+    {
+        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.x) |
+               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.y) << 16);
+    }
+};
+template <>
+struct Elem<FMT_CBF16> {
+    typedef uint32_t raw;
+    static __device__ __forceinline__ float2 f(raw w)
+    {
+        return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+    }
+    static __device__ __forceinline__ raw pack(float2 v)
+    {
+        return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.x) |
+               ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.y) << 16);
+    }
+};
+template <>
+struct Elem<FMT_C32> {
+    typedef float2 raw;
+    static __device__ __forceinline__ float2 f(raw w) { return w; }
+    static __device__ __forceinline__ raw pack(float2 v) { return v; }
+};
+
+// ------------------------------------------------------------------ geometry
+template <int PREC>
+constexpr int fast_waves() { return PREC == PREC_FP32 ? 4 : 8; }
+
+template <int R, int KIND, int NW_ = 4>
+struct Geo {
+    static constexpr int NW = NW_;            // waves per workgroup
+    static constexpr int NT = NW_ * 64;       // threads per workgroup
+    static constexpr int R16 = R / 16;
+    static constexpr int T = KIND == KIND_COLS ? 16 : (kMaxTile / R);
+    // ROWS*: [line][point], padded points and an odd line pitch (lanes walk either the
+    // points of a line or, in the transposed last stage, the lines);
+    // COLS: [point][16 lines] (every access walks the 16 lines of one point).
+    static constexpr int LP = R + R / 16 + 1;
+    static constexpr int SIZE = KIND == KIND_COLS ? R * 16 : T * LP;
+    static __device__ __forceinline__ int phys(int t, int p)
+    {
+        if constexpr (KIND == KIND_COLS)
+            return p * 16 + t;
+        else
+            return t * LP + p + (p >> 4);
+    }
+    // twiddle tables staged in LDS (0 = read from global memory)
+    static constexpr int STW = R <= 1024 ? R : 0;
+    static constexpr int PAIRS = R * T / 256;               // MFMA column groups per stage
+    static constexpr int MAXP = (PAIRS + NW - 1) / NW;
+};
+
+// slot (16 per MFMA group) -> (line t, column col) for a lane mapping
+template <int R, int T, int MAP>
+__device__ __forceinline__ void slot_tc(int slot, int &t, int &col)
+{
+    if constexpr (MAP == MAP_POINTS) {
+        t = slot / (R / 16);
+        col = slot % (R / 16);
+    } else {
+        t = slot % T;
+        col = slot / T;
+    }
+}
+
+struct Origin {
+    int l0b, l1, nv;
+    long long in_base, out_base;
+};
+
+__device__ __forceinline__ Origin origin(const TileArgs &a, int T, long long tile)
+{
+    int tiles0 = (a.c0 + T - 1) / T;
+    long long r = tile / tiles0;
+    int t0 = (int)(tile - r * tiles0);
+    int l1 = (int)(r % a.c1);
+    long long l2 = r / a.c1;
+    Origin o;
+    o.l0b = t0 * T;
+    o.l1 = l1;
+    o.nv = min(T, a.c0 - o.l0b);
+    o.in_base = (long long)o.l0b * a.in_s0 + (long long)l1 * a.in_s1 + l2 * a.in_s2;
+    o.out_base = (long long)o.l0b * a.out_s0 + (long long)l1 * a.out_s1 + l2 * a.out_s2;
+    return o;
+}
+
+// ------------------------------------------------------------------ DFT16 on MFMA
+// Per-lane A operands of the DFT16 (loaded once per kernel).
+template <int PREC>
+struct Ops {
+    h8 hr, hi, lr, li;
+    b8 br, bi;
+    __device__ __forceinline__ void load(const DftOps *ops, int lane)
+    {
+        if constexpr (PREC == PREC_BF16) {
+            br = *reinterpret_cast<const b8 *>(ops->br[lane]);
+            bi = *reinterpret_cast<const b8 *>(ops->bi[lane]);
+        } else {
+            hr = *reinterpret_cast<const h8 *>(ops->hr[lane]);
+            hi = *reinterpret_cast<const h8 *>(ops->hi_[lane]);
+            if constexpr (PREC == PREC_SPLIT) {
+                lr = *reinterpret_cast<const h8 *>(ops->lr[lane]);
+                li = *reinterpret_cast<const h8 *>(ops->li[lane]);
+            }
+        }
+    }
+};
+
+// x = 4 complex inputs of this lane (fp32), or `raw16` = the same inputs already in
+// fp16/bf16 (exact, from a 16-bit buffer). Returns Y[4g+i] of the lane's column (x16).
+template <int PREC, bool EXACT16>
+__device__ __forceinline__ void dft16(const float2 *x, const h8 &raw16, const b8 &rawb, const Ops<PREC> &op,
+                                      f4 &ar, f4 &ai)
+{
+    ar = f4{0.f, 0.f, 0.f, 0.f};
+    ai = ar;
+    if constexpr (PREC == PREC_BF16) {
+        b8 b;
+        if constexpr (EXACT16) {
+            b = rawb;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                b[2 * k] = (__bf16)x[k].x;
+                b[2 * k + 1] = (__bf16)x[k].y;
+            }
+        }
+        ar = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op.br, b, ar, 0, 0, 0);
+        ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op.bi, b, ai, 0, 0, 0);
+    } else {
+        h8 bh;
+        if constexpr (EXACT16) {
+            bh = raw16;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                bh[2 * k] = (_Float16)x[k].x;
+                bh[2 * k + 1] = (_Float16)x[k].y;
+            }
+        }
+        if constexpr (PREC == PREC_SPLIT) {
+            ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.lr, bh, ar, 0, 0, 0);
+            ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.li, bh, ai, 0, 0, 0);
+            if constexpr (!EXACT16) {
+                h8 bl;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    bl[2 * k] = (_Float16)(x[k].x - (float)bh[2 * k]);
+                    bl[2 * k + 1] = (_Float16)(x[k].y - (float)bh[2 * k + 1]);
+                }
+                ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hr, bl, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hi, bl, ai, 0, 0, 0);
+            }
+        }
+        ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hr, bh, ar, 0, 0, 0);
+        ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hi, bh, ai, 0, 0, 0);
+    }
+}
+
+// ------------------------------------------------------------------ first-stage loads
+// The raw (undecoded) first-stage operands of one tile, held in registers so the next
+// tile's loads can be in flight while the current tile is transformed.
+template <int PREC, int R, int KIND, int INF>
+struct Prefetch {
+    typedef Geo<R, KIND, fast_waves<PREC>()> G;
+    typedef typename Elem<INF>::raw raw;
+    static constexpr int N = PREC == PREC_FP32 ? 16 : 4 * G::MAXP;
+    raw w[N];
+
+    __device__ __forceinline__ void load(const TileArgs &a, const Origin &o)
+    {
+        constexpr int MAP = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
+        constexpr int R16 = G::R16;
+        const raw *src = reinterpret_cast<const raw *>(a.in);
+        if constexpr (PREC == PREC_FP32) {
+            int t, col;
+            slot_tc<R, G::T, MAP>(threadIdx.x, t, col);
+            const bool ok = t < o.nv;
+            const raw *base = src + o.in_base + (long long)t * a.in_s0 + (long long)col * a.in_ps;
+            const long long step = (long long)R16 * a.in_ps;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if constexpr (INF == FMT_C32)
+                    w[j] = ok ? base[j * step] : make_float2(0.f, 0.f);
+                else
+                    w[j] = ok ? base[j * step] : 0u;
+            }
+        } else {
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+            for (int u = 0; u < G::MAXP; ++u) {
+                const int P = wave + u * G::NW;
+                int t, col;
+                slot_tc<R, G::T, MAP>(P * 16 + c, t, col);
+                const bool ok = (G::PAIRS % G::NW == 0 || P < G::PAIRS) && t < o.nv;
+                const raw *base = src + o.in_base + (long long)t * a.in_s0 + (long long)(col + 4 * g * R16) * a.in_ps;
+                const long long step = (long long)R16 * a.in_ps;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if constexpr (INF == FMT_C32)
+                        w[4 * u + i] = ok ? base[i * step] : make_float2(0.f, 0.f);
+                    else
+                        w[4 * u + i] = ok ? base[i * step] : 0u;
+                }
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ radix-16 stage (MFMA)
+// FIRST: operands are the prefetched raw inputs; LAST: results go to global memory.
+// stw: stage twiddles (LDS or global, scale folded in); twlo/twhi: four-step tables.
+template <int PREC, int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF>
+__device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Origin &o, int s, const Ops<PREC> &op,
+                                       const Prefetch<PREC, R, KIND, INF> &pf, const float2 *stw,
+                                       const float2 *twlo, const float2 *twhi)
+{
+    typedef Geo<R, KIND, fast_waves<PREC>()> G;
+    constexpr int T = G::T, R16 = G::R16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    constexpr bool EXACT = FIRST && ((INF == FMT_C16 && PREC != PREC_BF16) || (INF == FMT_CBF16 && PREC == PREC_BF16));
+
+    f4 accr[G::MAXP], acci[G::MAXP];
+#pragma unroll
+    for (int u = 0; u < G::MAXP; ++u) {
+        const int P = wave + u * G::NW;
+        if (G::PAIRS % G::NW != 0 && P >= G::PAIRS) break;
+        int t, col;
+        slot_tc<R, T, MAP>(P * 16 + c, t, col);
+        float2 x[4];
+        h8 r16;
+        b8 rb;
+        if constexpr (FIRST) {
+            if constexpr (EXACT) {
+                u4 v = {pf.w[4 * u], pf.w[4 * u + 1], pf.w[4 * u + 2], pf.w[4 * u + 3]};
+                r16 = __builtin_bit_cast(h8, v);
+                rb = __builtin_bit_cast(b8, v);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) x[i] = Elem<INF>::f(pf.w[4 * u + i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = lds[G::phys(t, col + (4 * g + i) * R16)];
+        }
+        dft16<PREC, EXACT>(x, r16, rb, op, accr[u], acci[u]);
+    }
+    if constexpr (!FIRST && !LAST) __syncthreads();  // all reads of the tile precede the writes
+#pragma unroll
+    for (int u = 0; u < G::MAXP; ++u) {
+        const int P = wave + u * G::NW;
+        if (G::PAIRS % G::NW != 0 && P >= G::PAIRS) break;
+        int t, col;
+        slot_tc<R, T, MAP>(P * 16 + c, t, col);
+        if constexpr (!LAST) {
+            const int q = col % s, p = col / s;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int tq = 4 * g + i;
+                const float2 w = stw[p * tq * s];  // W_R^(p*tq*s), scale folded in
+                lds[G::phys(t, q + s * (16 * p + tq))] = cmulf(make_float2(accr[u][i], acci[u][i]), w);
+            }
+        } else {
+            if (t < o.nv) {
+                typedef typename Elem<OUTF>::raw raw;
+                raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
+                const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int pt = col + R16 * (4 * g + i);
+                    float2 y = make_float2(accr[u][i], acci[u][i]);
+                    if (a.tw_N) {
+                        const unsigned e = mult * (unsigned)pt;
+                        y = cmulf(y, cmulf(twhi[e >> a.tw_bits], twlo[e & ((1u << a.tw_bits) - 1u)]));
+                    } else {
+                        y.x *= a.last_mul;
+                        y.y *= a.last_mul;
+                    }
+                    dst[(long long)pt * a.out_ps] = Elem<OUTF>::pack(y);
+                }
+            }
+        }
+    }
+    if constexpr (!LAST) __syncthreads();
+}
+
+// ------------------------------------------------------------------ radix-16 stage (fp32 VALU)
+__device__ __forceinline__ void dft4v(float2 *v, int sign)
+{
+    float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    float2 d0 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    float2 s1 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    float2 d1 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    float2 r = make_float2(-sign * d1.y, sign * d1.x);
+    v[0] = make_float2(s0.x + s1.x, s0.y + s1.y);
+    v[1] = make_float2(d0.x + r.x, d0.y + r.y);
+    v[2] = make_float2(s0.x - s1.x, s0.y - s1.y);
+    v[3] = make_float2(d0.x - r.x, d0.y - r.y);
+}
+
+__device__ __forceinline__ void dft16v(float2 *v, int sign)
+{
+    float2 a[4][4];
+#pragma unroll
+    for (int j2 = 0; j2 < 4; ++j2) {
+        float2 t[4] = {v[j2], v[4 + j2], v[8 + j2], v[12 + j2]};
+        dft4v(t, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[k1][j2] = t[k1];
+    }
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+    const float wr[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
+    const float wi[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1)
+#pragma unroll
+        for (int j2 = 1; j2 < 4; ++j2) {
+            const int e = k1 * j2;
+            a[k1][j2] = cmulf(a[k1][j2], make_float2(wr[e], sign * wi[e]));
+        }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        dft4v(a[k1], sign);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = a[k1][k2];
+    }
+}
+
+// One thread = one 16-point column per stage (tile of 4096 points / 256 threads).
+template <int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF>
+__device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const Origin &o, int s,
+                                         const Prefetch<PREC_FP32, R, KIND, INF> &pf, const float2 *stw,
+                                         const float2 *twlo, const float2 *twhi)
+{
+    typedef Geo<R, KIND> G;
+    constexpr int T = G::T, R16 = G::R16;
+    static_assert(R * G::T / 16 == kThreads, "one column per thread");
+    int t, col;
+    slot_tc<R, T, MAP>(threadIdx.x, t, col);
+    float2 v[16];
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = Elem<INF>::f(pf.w[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = lds[G::phys(t, col + j * R16)];
+        if constexpr (!LAST) __syncthreads();
+    }
+    dft16v(v, a.sign);
+    if constexpr (!LAST) {
+        const int q = col % s, p = col / s;
+#pragma unroll
+        for (int tq = 0; tq < 16; ++tq)
+            lds[G::phys(t, q + s * (16 * p + tq))] = cmulf(v[tq], stw[p * tq * s]);
+        __syncthreads();
+    } else if (t < o.nv) {
+        typedef typename Elem<OUTF>::raw raw;
+        raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
+        const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+#pragma unroll
+        for (int tq = 0; tq < 16; ++tq) {
+            const int pt = col + R16 * tq;
+            float2 y = v[tq];
+            if (a.tw_N) {
+                const unsigned e = mult * (unsigned)pt;
+                y = cmulf(y, cmulf(twhi[e >> a.tw_bits], twlo[e & ((1u << a.tw_bits) - 1u)]));
+            } else {
+                y.x *= a.last_mul;
+                y.y *= a.last_mul;
+            }
+            dst[(long long)pt * a.out_ps] = Elem<OUTF>::pack(y);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ tail stage (radix 2/4/8, VALU)
+template <int RADIX>
+__device__ __forceinline__ void dft_tail(float2 *v, int sign)
+{
+    if constexpr (RADIX == 2) {
+        float2 x0 = v[0], x1 = v[1];
+        v[0] = make_float2(x0.x + x1.x, x0.y + x1.y);
+        v[1] = make_float2(x0.x - x1.x, x0.y - x1.y);
+    } else if constexpr (RADIX == 4) {
+        dft4v(v, sign);
+    } else {
+        const float h = 0.70710678118654752440f;
+        float2 a4[4], b4[4];
+#pragma unroll
+        for (int j2 = 0; j2 < 4; ++j2) {
+            float2 x0 = v[j2], x1 = v[4 + j2];
+            a4[j2] = make_float2(x0.x + x1.x, x0.y + x1.y);
+            b4[j2] = make_float2(x0.x - x1.x, x0.y - x1.y);
+        }
+        b4[1] = make_float2(h * (b4[1].x - sign * b4[1].y), h * (b4[1].y + sign * b4[1].x));
+        b4[2] = make_float2(-sign * b4[2].y, sign * b4[2].x);
+        b4[3] = make_float2(h * (-b4[3].x - sign * b4[3].y), h * (-b4[3].y + sign * b4[3].x));
+        dft4v(a4, sign);
+        dft4v(b4, sign);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            v[2 * k2] = a4[k2];
+            v[2 * k2 + 1] = b4[k2];
+        }
+    }
+}
+
+template <int R, int KIND, int RADIX, int NW>
+__device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, float mul)
+{
+    typedef Geo<R, KIND, NW> G;
+    constexpr int T = G::T;
+    constexpr int COLS = R / RADIX;
+    constexpr int ITEMS = COLS * T;
+    constexpr int NT = G::NT;
+    constexpr int PER = (ITEMS + NT - 1) / NT;
+    float2 v[PER][RADIX];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int I = threadIdx.x + k * NT;
+        if (ITEMS % NT != 0 && I >= ITEMS) break;
+        const int t = KIND == KIND_COLS ? I % T : I / COLS;
+        const int col = KIND == KIND_COLS ? I / T : I % COLS;
+#pragma unroll
+        for (int j = 0; j < RADIX; ++j) v[k][j] = lds[G::phys(t, col + j * COLS)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int I = threadIdx.x + k * NT;
+        if (ITEMS % NT != 0 && I >= ITEMS) break;
+        const int t = KIND == KIND_COLS ? I % T : I / COLS;
+        const int col = KIND == KIND_COLS ? I / T : I % COLS;
+        dft_tail<RADIX>(v[k], a.sign);
+        const int q = col % s, p = col / s;
+#pragma unroll
+        for (int tq = 0; tq < RADIX; ++tq) {
+            float2 y = make_float2(v[k][tq].x * mul, v[k][tq].y * mul);
+            if (p) y = cmulf(y, a.stage_tw[p * tq * s]);
+            lds[G::phys(t, q + s * (RADIX * p + tq))] = y;
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ kernel
+// R = 16^n * TAIL (TAIL in {1,2,4,8}, n >= 2): schedule [16, TAIL?, 16, ..., 16].
+// Persistent: workgroup b transforms tiles b, b + gridDim.x, ...; the first-stage
+// operands of the next tile are loaded before the current tile is transformed.
+template <int PREC, int R, int KIND, int INF, int OUTF>
+__global__ void __launch_bounds__(fast_waves<PREC>() * 64) fft_fast(TileArgs a)
+{
+    typedef Geo<R, KIND, fast_waves<PREC>()> G;
+    __shared__ float2 lds[G::SIZE];
+    __shared__ float2 s_stw[G::STW > 0 ? G::STW : 1];
+    __shared__ float2 s_tw[2 * kFastTw];
+    constexpr int L2 = __builtin_ctz(R);
+    constexpr int NS = L2 / 4;
+    constexpr int TAIL = 1 << (L2 % 4);
+    static_assert(NS >= 2, "fast kernels need R >= 256");
+    constexpr int MAP_IN = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
+    constexpr int MAP_OUT = KIND == KIND_ROWS ? MAP_POINTS : MAP_LINES;
+
+    // tables: stage twiddles and (small) four-step tables into LDS
+    const float2 *stw = a.stage_tw_s;
+    if constexpr (G::STW > 0) {
+        for (int i = threadIdx.x; i < R; i += G::NT) s_stw[i] = a.stage_tw_s[i];
+        stw = s_stw;
+    }
+    const float2 *twlo = a.tw_lo_s, *twhi = a.tw_hi;
+    if (a.tw_N && a.tw_lds) {
+        const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
+        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = a.tw_lo_s[i];
+        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[kFastTw + i] = a.tw_hi[i];
+        twlo = s_tw;
+        twhi = s_tw + kFastTw;
+    }
+    Ops<PREC> op;
+    if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
+    __syncthreads();
+
+    // Each of the 8 XCDs (workgroups are dealt to them round-robin) walks a contiguous
+    // range of tiles, so tiles that share cache lines meet in the same L2.
+    const long long ntiles = (long long)((a.c0 + G::T - 1) / G::T) * a.c1 * a.c2;
+    long long tile, step, end;
+    if ((gridDim.x & 7) == 0) {
+        const long long per = (ntiles + 7) >> 3;
+        const int xcd = blockIdx.x & 7;
+        tile = xcd * per + (blockIdx.x >> 3);
+        step = gridDim.x >> 3;
+        end = min(ntiles, (xcd + 1) * per);
+    } else {
+        tile = blockIdx.x;
+        step = gridDim.x;
+        end = ntiles;
+    }
+    Prefetch<PREC, R, KIND, INF> cur, nxt;
+    Origin o = origin(a, G::T, tile < end ? tile : 0);
+    if (tile < end) cur.load(a, o);
+    while (tile < end) {
+        const long long next = tile + step;
+        Origin on = o;
+        if (next < end) {
+            on = origin(a, G::T, next);
+            nxt.load(a, on);
+        }
+        int s = 1;
+        if constexpr (PREC == PREC_FP32)
+            vstage16<R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, cur, stw, twlo, twhi);
+        else
+            mstage<PREC, R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, op, cur, stw, twlo, twhi);
+        s = 16;
+        if constexpr (TAIL > 1) {
+            tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul);
+            s *= TAIL;
+        }
+#pragma unroll
+        for (int k = 1; k < NS - 1; ++k) {
+            if constexpr (PREC == PREC_FP32)
+                vstage16<R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, cur, stw, twlo, twhi);
+            else
+                mstage<PREC, R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, op, cur, stw, twlo, twhi);
+            s *= 16;
+        }
+        if constexpr (PREC == PREC_FP32)
+            vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, cur, stw, twlo, twhi);
+        else
+            mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, op, cur, stw, twlo, twhi);
+        __syncthreads();  // the next tile's first stage overwrites LDS
+        tile = next;
+        o = on;
+        cur = nxt;
+    }
+}
+
+template <int PREC, int KIND, int INF, int OUTF>
+fast_fn pick_r(int R)
+{
+    if constexpr (KIND == KIND_COLS) {
+        return R == 256 ? fft_fast<PREC, 256, KIND, INF, OUTF> : nullptr;
+    } else {
+        switch (R) {
+        case 256: return fft_fast<PREC, 256, KIND, INF, OUTF>;
+        case 512: return fft_fast<PREC, 512, KIND, INF, OUTF>;
+        case 1024: return fft_fast<PREC, 1024, KIND, INF, OUTF>;
+        case 2048: return fft_fast<PREC, 2048, KIND, INF, OUTF>;
+        case 4096: return fft_fast<PREC, 4096, KIND, INF, OUTF>;
+        default: return nullptr;
+        }
+    }
+}
+
+template <int PREC, int INF, int OUTF>
+fast_fn pick_kind(int R, int kind)
+{
+    switch (kind) {
+    case KIND_ROWS: return pick_r<PREC, KIND_ROWS, INF, OUTF>(R);
+    case KIND_ROWS_TR: return pick_r<PREC, KIND_ROWS_TR, INF, OUTF>(R);
+    case KIND_COLS: return pick_r<PREC, KIND_COLS, INF, OUTF>(R);
+    default: return nullptr;
+    }
+}
+
+}  // namespace fast
+}  // namespace mfft

@@ -10,6 +10,7 @@
 // workspace, processed in batch chunks small enough to stay in the 256 MB Infinity Cache.
 #include "../../include/mfft.h"
 #include "mfft_common.h"
+#include "mfft_fast.h"
 
 #include <algorithm>
 #include <cmath>
@@ -21,7 +22,8 @@
 #include <vector>
 
 namespace mfft {
-hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s);
+hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s,
+                       double scale = 1.0);
 hipError_t gen_dft_ops(DftOps *ops, int sign, hipStream_t s);
 typedef void (*tile_fn)(TileArgs);
 tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
@@ -123,6 +125,10 @@ struct PassSpec {
     // derived
     std::vector<int> radix;
     bool generic = false;
+    int fast_kind = -1;   // KIND_* of the fast kernel, -1 = tile kernel
+    float2 *stage_tw_s = nullptr, *tw_lo_s = nullptr;
+    float last_mul = 1.f, tail_mul = 1.f;
+    long long max_grid = 0;  // persistent grid of the fast kernel
     int TL = 1, T = 1;
     int l0 = 0, l1 = -1, l2 = -1;  // indices into lines
     const void *fn = nullptr;
@@ -268,7 +274,51 @@ static int finish_passes(mfft_plan_s *p)
             pow2 = false;
         }
         ps.generic = !pow2;
-        if (!ps.generic) {
+        // fast kernel: power of two 256..4096, no fused permutation
+        if (pow2 && ps.R >= 256 && ps.R <= kMaxTile && ps.perm_axis < 0) {
+            int kind = -1, l0 = -1;
+            int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
+            int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+            if (ps.ps_in == 1 && ps.ps_out == 1) {
+                kind = KIND_ROWS;
+                l0 = 0;
+                for (int i = 1; i < (int)ps.lines.size(); ++i)
+                    if (ps.lines[i].in_stride < ps.lines[l0].in_stride) l0 = i;
+            } else if (ps.ps_in == 1 && unit_out >= 0) {
+                kind = KIND_ROWS_TR;
+                l0 = unit_out;
+            } else if (ps.ps_in != 1 && ps.R == 256) {
+                for (int i = 0; i < (int)ps.lines.size(); ++i)
+                    if (ps.lines[i].in_stride == 1 && ps.lines[i].out_stride == 1) { l0 = i; break; }
+                if (l0 >= 0) kind = KIND_COLS;
+            }
+            const void *fn = kind >= 0 ? (const void *)find_fast_kernel(p->d.precision, ps.R, kind, in_fmt, out_fmt) : nullptr;
+            if (fn) {
+                ps.fast_kind = kind;
+                ps.l0 = l0;
+                ps.T = fast_tile_lines(ps.R, kind);
+                ps.TL = kind == KIND_COLS ? 16 : 1;
+                ps.fn = fn;
+                int dev = 0, ncu = 0, nb = 0;
+                HIP_TRY(hipGetDevice(&dev));
+                HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, fast_threads(p->d.precision), 0));
+                ps.max_grid = (long long)std::max(1, nb) * std::max(1, ncu);
+                ps.max_grid -= ps.max_grid % 8;
+                if (ps.max_grid < 8) ps.max_grid = 8;
+                // radix order of the fast schedule: 16, tail, 16, ...
+                std::vector<int> r2;
+                int tail = 1;
+                for (int r : ps.radix) if (r != 16) tail = r;
+                r2.push_back(16);
+                if (tail > 1) r2.push_back(tail);
+                for (size_t i = 1; i < ps.radix.size() - (tail > 1 ? 1 : 0); ++i) r2.push_back(16);
+                ps.radix = r2;
+            }
+        }
+        if (ps.fast_kind >= 0) {
+            // l0 chosen above
+        } else if (!ps.generic) {
             ps.T = pow2_tile_lines(ps.R, ps.TL);
         } else {
             int t = kMaxTile / ps.R;
@@ -287,8 +337,10 @@ static int finish_passes(mfft_plan_s *p)
         ps.l2 = rest.size() > 1 ? rest[1] : -1;
         if (ps.tw_N && ps.tw_level != ps.l0 && ps.tw_level != ps.l1)
             return fail(MFFT_ERR_UNSUPPORTED, "twiddle level layout");
-        ps.fn = (const void *)find_tile_kernel(p->d.precision, ps.R, ps.TL, ps.generic);
-        if (!ps.fn) return fail(MFFT_ERR_UNSUPPORTED, "no kernel for R=%d TL=%d", ps.R, ps.TL);
+        if (ps.fast_kind < 0) {
+            ps.fn = (const void *)find_tile_kernel(p->d.precision, ps.R, ps.TL, ps.generic);
+            if (!ps.fn) return fail(MFFT_ERR_UNSUPPORTED, "no kernel for R=%d TL=%d", ps.R, ps.TL);
+        }
     }
     return MFFT_OK;
 }
@@ -319,6 +371,23 @@ static int build_tables(mfft_plan_s *p)
             HIP_TRY(gen_twiddle(ps.tw_lo, ps.tw_N, nlo, 1, sign, 0));
             HIP_TRY(gen_twiddle(ps.tw_hi, ps.tw_N, nhi, nlo, sign, 0));
         }
+        if (ps.fast_kind >= 0) {
+            // radix-16 stage scale (1/16 per stage when scaling), divided by the factor 16
+            // carried by the matrix-core operands; the fp32 path has no such factor
+            double fac = (p->d.scale == MFFT_SCALE_STAGE ? 1.0 / 16 : 1.0) /
+                         (p->d.precision == MFFT_PREC_FP32 ? 1.0 : 16.0);
+            ps.last_mul = (float)fac;
+            int tail = 1;
+            for (int r : ps.radix) if (r != 16) tail = r;
+            ps.tail_mul = (p->d.scale == MFFT_SCALE_STAGE) ? 1.0f / tail : 1.0f;
+            if (alloc_dev(p, (void **)&ps.stage_tw_s, sizeof(float2) * ps.R)) return MFFT_ERR_NOMEM;
+            HIP_TRY(gen_twiddle(ps.stage_tw_s, ps.R, ps.R, 1, sign, 0, fac));
+            if (ps.tw_N) {
+                long long nlo = 1LL << ps.tw_bits;
+                if (alloc_dev(p, (void **)&ps.tw_lo_s, sizeof(float2) * nlo)) return MFFT_ERR_NOMEM;
+                HIP_TRY(gen_twiddle(ps.tw_lo_s, ps.tw_N, nlo, 1, sign, 0, fac));
+            }
+        }
         if (ps.perm_axis >= 0 && p->d.in_order == MFFT_ORDER_DIGIT_REVERSED) {
             const std::vector<int> &rev = p->rev_axis[ps.perm_axis];
             if ((int)rev.size() != ps.R) return fail(MFFT_ERR_INVALID, "permutation size");
@@ -332,6 +401,9 @@ static int build_tables(mfft_plan_s *p)
 
 static int finalize_plan(mfft_plan_s *p)
 {
+    p->mid_fmt = (p->d.precision == MFFT_PREC_FP16) ? FMT_C16
+                 : (p->d.precision == MFFT_PREC_BF16) ? FMT_CBF16
+                                                      : FMT_C32;
     int rc = finish_passes(p);
     if (rc) return rc;
     // scale bookkeeping
@@ -340,14 +412,11 @@ static int finalize_plan(mfft_plan_s *p)
         for (int r : ps.radix) sc /= (p->d.scale == MFFT_SCALE_STAGE ? r : 1);
     p->out_scale = sc;
     // workspace and batch chunking
-    p->mid_fmt = (p->d.precision == MFFT_PREC_FP16) ? FMT_C16
-                 : (p->d.precision == MFFT_PREC_BF16) ? FMT_CBF16
-                                                      : FMT_C32;
     bool need_work = false;
     for (auto &ps : p->passes)
         if (ps.src == BUF_WORK || ps.dst == BUF_WORK) need_work = true;
     long long per = p->total * fmt_bytes(p->mid_fmt);
-    long long limit = p->d.workspace_limit > 0 ? p->d.workspace_limit : (64LL << 20);
+    long long limit = p->d.workspace_limit > 0 ? p->d.workspace_limit : (128LL << 20);
     p->chunk = p->batch;
     if (need_work) {
         long long c = std::max(1LL, limit / std::max(1LL, per));
@@ -375,7 +444,8 @@ static void *buf_ptr(int which, const void *in, void *out, void *work)
     return which == BUF_IN ? const_cast<void *>(in) : which == BUF_OUT ? out : work;
 }
 
-static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s)
+static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s, hipEvent_t *ev,
+                      int &evi)
 {
     for (auto &ps : p->passes) {
         TileArgs a;
@@ -419,10 +489,23 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         a.sign = p->d.direction;
         a.in_vec = ((uintptr_t)a.in & 15) == 0;
         a.out_vec = ((uintptr_t)a.out & 15) == 0;
+        a.stage_tw_s = ps.stage_tw_s;
+        a.tw_lo_s = ps.tw_lo_s;
+        a.last_mul = ps.last_mul;
+        a.tail_mul = ps.tail_mul;
         long long ntiles = ((a.c0 + a.T - 1) / a.T) * (long long)a.c1 * a.c2;
         if (ntiles <= 0) continue;
         if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
-        HIP_TRY(launch_tile(ps.fn, a, ntiles, s));
+        if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
+        if (ps.fast_kind >= 0) {
+            a.tw_lds = ps.tw_N && (1 << ps.tw_bits) <= kFastTw && ((ps.tw_N + (1 << ps.tw_bits) - 1) >> ps.tw_bits) <= kFastTw;
+            long long grid = std::min(ntiles, ps.max_grid);
+            if (grid > 8) grid -= grid % 8;
+            void *args[] = {&a};
+            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)grid), dim3(fast_threads(p->d.precision)), args, 0, s));
+        } else {
+            HIP_TRY(launch_tile(ps.fn, a, ntiles, s));
+        }
     }
     return MFFT_OK;
 }
@@ -551,6 +634,12 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
         long long c0 = ps.lines[ps.l0].batch ? p->chunk : ps.lines[ps.l0].count;
         pi.tiles = (lines / c0) * ((c0 + ps.T - 1) / ps.T);
         pi.generic = ps.generic;
+        auto fmt_of = [&](int b) {
+            return b == BUF_IN ? p->d.in_fmt : b == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
+        };
+        pi.in_elem_bytes = fmt_bytes(fmt_of(ps.src));
+        pi.out_elem_bytes = fmt_bytes(fmt_of(ps.dst));
+        pi.points = lines * ps.R;
     }
     info->out_scale = p->out_scale;
     info->workspace_bytes = p->work_bytes;
@@ -559,18 +648,34 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
     return MFFT_OK;
 }
 
-int mfft_exec(mfft_plan p, const void *in, void *out, void *stream)
+static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEvent_t *ev)
 {
-    if (!p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(p->exec_mu);
     hipStream_t s = (hipStream_t)stream;
     long long ib = p->total * fmt_bytes(p->d.in_fmt), ob = p->total * fmt_bytes(p->d.out_fmt);
+    int evi = 0;
     for (long long b0 = 0; b0 < p->batch; b0 += p->chunk) {
         long long nb = std::min(p->chunk, p->batch - b0);
-        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s);
+        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi);
         if (rc) return rc;
     }
+    if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
     return MFFT_OK;
+}
+
+int mfft_exec(mfft_plan p, const void *in, void *out, void *stream)
+{
+    if (!p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
+    return exec_all(p, in, out, stream, nullptr);
+}
+
+int mfft_exec_events(mfft_plan p, const void *in, void *out, void *stream, void **events, int nevents)
+{
+    if (!p || !in || !out || !events) return fail(MFFT_ERR_INVALID, "null argument");
+    long long nchunks = (p->batch + p->chunk - 1) / p->chunk;
+    long long need = nchunks * (long long)p->passes.size() + 1;
+    if (nevents < need) return fail(MFFT_ERR_INVALID, "need %lld events, got %d", need, nevents);
+    return exec_all(p, in, out, stream, (hipEvent_t *)events);
 }
 
 int mfft_input_permutation(mfft_plan p, int axis, int *rev, int n)

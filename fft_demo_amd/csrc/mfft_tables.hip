@@ -9,15 +9,16 @@
 
 namespace mfft {
 
-// out[i] = exp(sign * 2*pi*i * ((i * mult) mod n) / n), i in [0, count).
-__global__ void gen_twiddle_kernel(float2 *out, long long n, long long count, long long mult, int sign)
+// out[i] = scale * exp(sign * 2*pi*i * ((i * mult) mod n) / n), i in [0, count); the
+// scale (a power of two) is applied in FP64 before the single rounding to fp32.
+__global__ void gen_twiddle_kernel(float2 *out, long long n, long long count, long long mult, int sign, double scale)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     long long e = (long long)(((__int128)i * mult) % n);
     double sn, cs;
     sincospi(2.0 * (double)e / (double)n, &sn, &cs);
-    out[i] = make_float2((float)cs, (float)(sign * sn));
+    out[i] = make_float2((float)(scale * cs), (float)(scale * sign * sn));
 }
 
 // DFT16 operands for lane l: rows m = l % 16 of the K-folded real matrix, K entries
@@ -48,11 +49,12 @@ __global__ void gen_dft_ops_kernel(DftOps *ops, int sign)
     }
 }
 
-hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s)
+hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s,
+                       double scale)
 {
     if (count <= 0) return hipSuccess;
     unsigned blocks = (unsigned)((count + 255) / 256);
-    hipLaunchKernelGGL(gen_twiddle_kernel, dim3(blocks), dim3(256), 0, s, out, n, count, mult, sign);
+    hipLaunchKernelGGL(gen_twiddle_kernel, dim3(blocks), dim3(256), 0, s, out, n, count, mult, sign, scale);
     return hipGetLastError();
 }
 

@@ -411,29 +411,60 @@ __device__ void copy_in(float2 *lds, const Geo &G, const TileArgs &a, const Tile
             lds[G.phys(t, p)] = make_float2(0.f, 0.f);
         }
     }
+    // Every thread first issues all of its global loads (UNR vectors), then scatters them
+    // into LDS, so a tile's loads are in flight together.
+    constexpr int UNR = 4;
     if (a.in_vec && a.in_ps == 1 && (R & 3) == 0 && (a.in_s0 & 3) == 0 && (o.in_base & 3) == 0) {
         // vectors of 4 consecutive points of one line
         const int per_line = R >> 2;
-        for (int ch = threadIdx.x; ch < nv * per_line; ch += kThreads) {
-            int t = ch / per_line, p0 = (ch - t * per_line) << 2;
-            float2 v[4];
-            cload4(a.in, o.in_base + (long long)t * a.in_s0 + p0, a.in_fmt, v);
+        const int nch = nv * per_line;
+        for (int base = threadIdx.x; base < nch; base += UNR * kThreads) {
+            float2 v[UNR][4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int pt = perm ? perm[p0 + k] : p0 + k;
-                lds[G.phys(t, pt)] = v[k];
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int t = ch / per_line, p0 = (ch - t * per_line) << 2;
+                    cload4(a.in, o.in_base + (long long)t * a.in_s0 + p0, a.in_fmt, v[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int t = ch / per_line, p0 = (ch - t * per_line) << 2;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        int pt = perm ? perm[p0 + k] : p0 + k;
+                        lds[G.phys(t, pt)] = v[u][k];
+                    }
+                }
             }
         }
     } else if (a.in_vec && a.in_s0 == 1 && (nv & 3) == 0 && (o.in_base & 3) == 0 && (a.in_ps & 3) == 0) {
         // vectors of 4 consecutive lines at one point
         const int per_pt = nv >> 2;
-        for (int ch = threadIdx.x; ch < R * per_pt; ch += kThreads) {
-            int p = ch / per_pt, t0 = (ch - p * per_pt) << 2;
-            float2 v[4];
-            cload4(a.in, o.in_base + (long long)p * a.in_ps + t0, a.in_fmt, v);
-            int pt = perm ? perm[p] : p;
+        const int nch = R * per_pt;
+        for (int base = threadIdx.x; base < nch; base += UNR * kThreads) {
+            float2 v[UNR][4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lds[G.phys(t0 + k, pt)] = v[k];
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int p = ch / per_pt, t0 = (ch - p * per_pt) << 2;
+                    cload4(a.in, o.in_base + (long long)p * a.in_ps + t0, a.in_fmt, v[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int p = ch / per_pt, t0 = (ch - p * per_pt) << 2;
+                    int pt = perm ? perm[p] : p;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) lds[G.phys(t0 + k, pt)] = v[u][k];
+                }
+            }
         }
     } else {
         for (int e = threadIdx.x; e < nv * R; e += kThreads) {
@@ -465,23 +496,52 @@ __device__ void copy_out(const float2 *lds, const Geo &G, const TileArgs &a, con
 {
     const int R = G.R;
     const int nv = o.nvalid;
+    constexpr int UNR = 4;
     if (a.out_vec && a.out_ps == 1 && (R & 3) == 0 && (a.out_s0 & 3) == 0 && (o.out_base & 3) == 0) {
         const int per_line = R >> 2;
-        for (int ch = threadIdx.x; ch < nv * per_line; ch += kThreads) {
-            int t = ch / per_line, p0 = (ch - t * per_line) << 2;
-            float2 v[4];
+        const int nch = nv * per_line;
+        for (int base = threadIdx.x; base < nch; base += UNR * kThreads) {
+            float2 v[UNR][4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = out_value(lds, G, a, o.l0b, o.l1, t, p0 + k);
-            cstore4(a.out, o.out_base + (long long)t * a.out_s0 + p0, a.out_fmt, v);
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int t = ch / per_line, p0 = (ch - t * per_line) << 2;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[u][k] = out_value(lds, G, a, o.l0b, o.l1, t, p0 + k);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int t = ch / per_line, p0 = (ch - t * per_line) << 2;
+                    cstore4(a.out, o.out_base + (long long)t * a.out_s0 + p0, a.out_fmt, v[u]);
+                }
+            }
         }
     } else if (a.out_vec && a.out_s0 == 1 && (nv & 3) == 0 && (o.out_base & 3) == 0 && (a.out_ps & 3) == 0) {
         const int per_pt = nv >> 2;
-        for (int ch = threadIdx.x; ch < R * per_pt; ch += kThreads) {
-            int p = ch / per_pt, t0 = (ch - p * per_pt) << 2;
-            float2 v[4];
+        const int nch = R * per_pt;
+        for (int base = threadIdx.x; base < nch; base += UNR * kThreads) {
+            float2 v[UNR][4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = out_value(lds, G, a, o.l0b, o.l1, t0 + k, p);
-            cstore4(a.out, o.out_base + (long long)p * a.out_ps + t0, a.out_fmt, v);
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int p = ch / per_pt, t0 = (ch - p * per_pt) << 2;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[u][k] = out_value(lds, G, a, o.l0b, o.l1, t0 + k, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                int ch = base + u * kThreads;
+                if (ch < nch) {
+                    int p = ch / per_pt, t0 = (ch - p * per_pt) << 2;
+                    cstore4(a.out, o.out_base + (long long)p * a.out_ps + t0, a.out_fmt, v[u]);
+                }
+            }
         }
     } else {
         for (int e = threadIdx.x; e < nv * R; e += kThreads) {

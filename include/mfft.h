@@ -77,6 +77,8 @@ typedef struct mfft_pass_info {
     int radix[8];
     int64_t tiles;  /* workgroups per launch (per batch chunk) */
     int generic;    /* 1 = runtime-schedule kernel (odd radices) */
+    int in_elem_bytes, out_elem_bytes; /* bytes per complex element read / written */
+    int64_t points; /* complex elements read (= written) per launch of a full chunk */
 } mfft_pass_info;
 
 typedef struct mfft_info {
@@ -107,6 +109,11 @@ int mfft_plan_info(mfft_plan plan, mfft_info *info);
  * tcfftExec (tcfft_half.cu:1177-1233), tcfftExec_2d (tcfft_half_2d.cu:1128-1149) and
  * TensorCoreFFTExec (tensorcore_fft_conv2d_kernel.cu:1495-1525). */
 int mfft_exec(mfft_plan plan, const void *in, void *out, void *stream);
+
+/* As mfft_exec, and records hipEvent_t events[k] on the stream before launch k and
+ * events[launches] after the last one (nevents >= launches_per_exec + 1), so a caller
+ * can time every kernel of the exec inside its own timed region. */
+int mfft_exec_events(mfft_plan plan, const void *in, void *out, void *stream, void **events, int nevents);
 
 /* Digit-reversal table of the plan's radix list (gen_rev, tcfft_half.cu:1351-1374):
  * position i of a digit-reversed buffer holds natural element rev[i]. For 2D plans,
