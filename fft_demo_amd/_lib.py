@@ -40,6 +40,7 @@ class Desc(ctypes.Structure):
         ("in_order", ctypes.c_int),
         ("workspace_limit", ctypes.c_int64),
         ("split_hint", ctypes.c_int),
+        ("mid_fmt", ctypes.c_int),
     ]
 
 
@@ -117,9 +118,51 @@ SIGNATURES = {
     "mfft_exec": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mfft_exec_events": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int]),
     "mfft_input_permutation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int]),
+    "mfft_stage_convert": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _i64, _i64, _vp, ctypes.c_int, _vp,
+                                          ctypes.c_int, ctypes.c_double, _vp]),
     "mfft_last_error": (ctypes.c_char_p, []),
     "mfft_version": (ctypes.c_char_p, []),
 }
+
+
+class TcfftHandle(ctypes.Structure):
+    """struct tcfftHandle of include/tcfft_half.h (reference tcfft_half.h:7-18)."""
+    _fields_ = [("N", ctypes.c_int), ("N_batch", ctypes.c_int), ("radices", ctypes.c_int * 9),
+                ("n_radices", ctypes.c_int), ("mergings", ctypes.c_int * 3), ("n_mergings", ctypes.c_int),
+                ("impl", ctypes.c_void_p)]
+
+
+class TcfftHandle2d(ctypes.Structure):
+    """struct tcfftHandle_2d of include/tcfft_half_2d.h (reference tcfft_half_2d.h:7-18)."""
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("N_batch", ctypes.c_int),
+                ("radices_x", ctypes.c_int * 3), ("radices_y", ctypes.c_int * 3), ("n_radices_x", ctypes.c_int),
+                ("n_radices_y", ctypes.c_int), ("mergings", ctypes.c_int * 2), ("impl", ctypes.c_void_p)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES.update({
+    # include/tcfft_half.h
+    "tcfftCreate": (None, [ctypes.POINTER(TcfftHandle), ctypes.c_int, ctypes.c_int]),
+    "tcfftExec": (None, [TcfftHandle, _vp]),
+    "tcfftDestroy": (None, [ctypes.POINTER(TcfftHandle)]),
+    "tcfftGetLastError": (ctypes.c_int, []),
+    "setup": (None, [_dp, ctypes.c_int, ctypes.c_int]),
+    "doit": (None, [ctypes.c_int]),
+    "finalize": (None, [_dp, ctypes.c_int]),
+    "setup_cu": (None, [_dp, ctypes.c_int, ctypes.c_int]),
+    "doit_cu": (None, [ctypes.c_int]),
+    "finalize_cu": (None, [_dp, ctypes.c_int, ctypes.c_int]),
+    "setup_cu_2d": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "doit_cu_2d": (None, [ctypes.c_int]),
+    "finalize_cu_2d": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    # include/tcfft_half_2d.h
+    "tcfftCreate_2d": (None, [ctypes.POINTER(TcfftHandle2d), ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "tcfftExec_2d": (None, [TcfftHandle2d, _vp]),
+    "tcfftDestroy_2d": (None, [ctypes.POINTER(TcfftHandle2d)]),
+    "setup_2d": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "doit_2d": (None, [ctypes.c_int]),
+    "finalize_2d": (None, [_dp, ctypes.c_int]),
+})
 
 
 def load(path: str | None = None):

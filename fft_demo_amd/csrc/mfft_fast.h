@@ -19,10 +19,11 @@
 namespace mfft {
 
 enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2 };
+constexpr int kColsT = 32;  // lines per tile of the strided kind
 
 typedef void (*fast_fn)(TileArgs);
 fast_fn find_fast_kernel(int prec, int R, int kind, int in_fmt, int out_fmt);
 int fast_tile_lines(int R, int kind);
-int fast_threads(int prec);
+int fast_threads(int prec, int kind);
 
 }  // namespace mfft

@@ -13,6 +13,9 @@ fast_fn find_fast_kernel(int prec, int R, int kind, int inf, int outf)
     return nullptr;
 }
 
-int fast_tile_lines(int R, int kind) { return kind == KIND_COLS ? 16 : kMaxTile / R; }
-int fast_threads(int prec) { return prec == PREC_FP32 ? 256 : 512; }
+int fast_tile_lines(int R, int kind) { return kind == KIND_COLS ? kColsT : kMaxTile / R; }
+int fast_threads(int prec, int kind)
+{
+    return (prec == PREC_FP32 ? 256 : 512) * (kind == KIND_COLS ? kColsT / 16 : 1);
+}
 }  // namespace mfft

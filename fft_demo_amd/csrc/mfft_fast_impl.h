@@ -56,24 +56,25 @@ struct Elem<FMT_C32> {
 };
 
 // ------------------------------------------------------------------ geometry
-template <int PREC>
-constexpr int fast_waves() { return PREC == PREC_FP32 ? 4 : 8; }
+// Waves per workgroup: the strided kind holds kColsT lines per tile (twice the points).
+template <int PREC, int KIND>
+constexpr int fast_waves() { return (PREC == PREC_FP32 ? 4 : 8) * (KIND == KIND_COLS ? kColsT / 16 : 1); }
 
 template <int R, int KIND, int NW_ = 4>
 struct Geo {
     static constexpr int NW = NW_;            // waves per workgroup
     static constexpr int NT = NW_ * 64;       // threads per workgroup
     static constexpr int R16 = R / 16;
-    static constexpr int T = KIND == KIND_COLS ? 16 : (kMaxTile / R);
+    static constexpr int T = KIND == KIND_COLS ? kColsT : (kMaxTile / R);
     // ROWS*: [line][point], padded points and an odd line pitch (lanes walk either the
     // points of a line or, in the transposed last stage, the lines);
-    // COLS: [point][16 lines] (every access walks the 16 lines of one point).
+    // COLS: [point][T lines] (every access walks consecutive lines of one point).
     static constexpr int LP = R + R / 16 + 1;
-    static constexpr int SIZE = KIND == KIND_COLS ? R * 16 : T * LP;
+    static constexpr int SIZE = KIND == KIND_COLS ? R * T : T * LP;
     static __device__ __forceinline__ int phys(int t, int p)
     {
         if constexpr (KIND == KIND_COLS)
-            return p * 16 + t;
+            return p * T + t;
         else
             return t * LP + p + (p >> 4);
     }
@@ -195,7 +196,7 @@ __device__ __forceinline__ void dft16(const float2 *x, const h8 &raw16, const b8
 // tile's loads can be in flight while the current tile is transformed.
 template <int PREC, int R, int KIND, int INF>
 struct Prefetch {
-    typedef Geo<R, KIND, fast_waves<PREC>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
     typedef typename Elem<INF>::raw raw;
     static constexpr int N = PREC == PREC_FP32 ? 16 : 4 * G::MAXP;
     raw w[N];
@@ -249,7 +250,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
                                        const Prefetch<PREC, R, KIND, INF> &pf, const float2 *stw,
                                        const float2 *twlo, const float2 *twhi)
 {
-    typedef Geo<R, KIND, fast_waves<PREC>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
     constexpr int T = G::T, R16 = G::R16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -367,9 +368,9 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
                                          const Prefetch<PREC_FP32, R, KIND, INF> &pf, const float2 *stw,
                                          const float2 *twlo, const float2 *twhi)
 {
-    typedef Geo<R, KIND> G;
+    typedef Geo<R, KIND, fast_waves<PREC_FP32, KIND>()> G;
     constexpr int T = G::T, R16 = G::R16;
-    static_assert(R * G::T / 16 == kThreads, "one column per thread");
+    static_assert(R * G::T / 16 == G::NT, "one column per thread");
     int t, col;
     slot_tc<R, T, MAP>(threadIdx.x, t, col);
     float2 v[16];
@@ -483,9 +484,9 @@ __device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, fl
 // Persistent: workgroup b transforms tiles b, b + gridDim.x, ...; the first-stage
 // operands of the next tile are loaded before the current tile is transformed.
 template <int PREC, int R, int KIND, int INF, int OUTF>
-__global__ void __launch_bounds__(fast_waves<PREC>() * 64) fft_fast(TileArgs a)
+__global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(TileArgs a)
 {
-    typedef Geo<R, KIND, fast_waves<PREC>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
     __shared__ float2 lds[G::SIZE];
     __shared__ float2 s_stw[G::STW > 0 ? G::STW : 1];
     __shared__ float2 s_tw[2 * kFastTw];

@@ -29,6 +29,8 @@ typedef void (*tile_fn)(TileArgs);
 tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
 int pow2_tile_lines(int R, int TL);
 hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream);
+hipError_t stage_convert(const void *src, int sfmt, void *dst, int dfmt, long long n_inner, long long batch,
+                         const int *perm_x, int nx, const int *perm_y, int ny, double scale, hipStream_t s);
 }  // namespace mfft
 
 using namespace mfft;
@@ -122,6 +124,7 @@ struct PassSpec {
     int tw_level = -1;  // index into lines
     int src = BUF_IN, dst = BUF_OUT;
     int perm_axis = -1; // axis whose digit-reversal is fused into this pass's loads
+    long long work_stride = 0;  // workspace elements per transform when padded (0 = dense)
     // derived
     std::vector<int> radix;
     bool generic = false;
@@ -297,12 +300,12 @@ static int finish_passes(mfft_plan_s *p)
                 ps.fast_kind = kind;
                 ps.l0 = l0;
                 ps.T = fast_tile_lines(ps.R, kind);
-                ps.TL = kind == KIND_COLS ? 16 : 1;
+                ps.TL = kind == KIND_COLS ? kColsT : 1;
                 ps.fn = fn;
                 int dev = 0, ncu = 0, nb = 0;
                 HIP_TRY(hipGetDevice(&dev));
                 HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, fast_threads(p->d.precision), 0));
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, fast_threads(p->d.precision, kind), 0));
                 ps.max_grid = (long long)std::max(1, nb) * std::max(1, ncu);
                 ps.max_grid -= ps.max_grid % 8;
                 if (ps.max_grid < 8) ps.max_grid = 8;
@@ -404,6 +407,7 @@ static int finalize_plan(mfft_plan_s *p)
     p->mid_fmt = (p->d.precision == MFFT_PREC_FP16) ? FMT_C16
                  : (p->d.precision == MFFT_PREC_BF16) ? FMT_CBF16
                                                       : FMT_C32;
+    if (p->d.mid_fmt >= 0) p->mid_fmt = p->d.mid_fmt;
     int rc = finish_passes(p);
     if (rc) return rc;
     // scale bookkeeping
@@ -415,7 +419,9 @@ static int finalize_plan(mfft_plan_s *p)
     bool need_work = false;
     for (auto &ps : p->passes)
         if (ps.src == BUF_WORK || ps.dst == BUF_WORK) need_work = true;
-    long long per = p->total * fmt_bytes(p->mid_fmt);
+    long long wper = p->total;
+    for (auto &ps : p->passes) wper = std::max(wper, ps.work_stride);
+    long long per = wper * fmt_bytes(p->mid_fmt);
     long long limit = p->d.workspace_limit > 0 ? p->d.workspace_limit : (128LL << 20);
     p->chunk = p->batch;
     if (need_work) {
@@ -435,6 +441,7 @@ static int check_desc(const mfft_desc *d)
     if (d->in_fmt < 0 || d->in_fmt > 2 || d->out_fmt < 0 || d->out_fmt > 2)
         return fail(MFFT_ERR_INVALID, "format");
     if (d->in_order != 0 && d->in_order != 1) return fail(MFFT_ERR_INVALID, "in_order");
+    if (d->mid_fmt < -1 || d->mid_fmt > 2) return fail(MFFT_ERR_INVALID, "mid_fmt");
     return MFFT_OK;
 }
 
@@ -502,7 +509,7 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             long long grid = std::min(ntiles, ps.max_grid);
             if (grid > 8) grid -= grid % 8;
             void *args[] = {&a};
-            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)grid), dim3(fast_threads(p->d.precision)), args, 0, s));
+            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)grid), dim3(fast_threads(p->d.precision, ps.fast_kind)), args, 0, s));
         } else {
             HIP_TRY(launch_tile(ps.fn, a, ntiles, s));
         }
@@ -523,6 +530,7 @@ int mfft_desc_default(mfft_desc *d)
     d->in_fmt = MFFT_FMT_C16;
     d->out_fmt = MFFT_FMT_C16;
     d->in_order = MFFT_ORDER_NATURAL;
+    d->mid_fmt = -1;
     return MFFT_OK;
 }
 
@@ -687,6 +695,18 @@ int mfft_input_permutation(mfft_plan p, int axis, int *rev, int n)
     rev_radices(len, rad);
     gen_rev(len, rad, r);
     memcpy(rev, r.data(), sizeof(int) * len);
+    return MFFT_OK;
+}
+
+int mfft_stage_convert(const void *src, int src_fmt, void *dst, int dst_fmt, int64_t n_inner, int64_t batch,
+                       const int *perm_x, int nx, const int *perm_y, int ny, double scale, void *stream)
+{
+    if (!src || !dst || n_inner < 0 || batch < 0) return fail(MFFT_ERR_INVALID, "bad argument");
+    if (src_fmt < 0 || src_fmt > 3 || dst_fmt < 0 || dst_fmt > 3) return fail(MFFT_ERR_INVALID, "format");
+    if (perm_y && (ny <= 0 || n_inner % ny != 0)) return fail(MFFT_ERR_INVALID, "2D staging needs ny | n_inner");
+    if (src == dst) return fail(MFFT_ERR_INVALID, "staging is out of place");
+    HIP_TRY(stage_convert(src, src_fmt, dst, dst_fmt, n_inner, batch, perm_x, nx, perm_y, ny, scale,
+                          (hipStream_t)stream));
     return MFFT_OK;
 }
 

@@ -33,7 +33,7 @@ def _enum(table: dict, v):
 
 
 def make_desc(precision="split_fp16", direction=L.FORWARD, scale=None, in_fmt=None, out_fmt=None,
-              in_order=L.ORDER_NATURAL, workspace_limit=0, split_hint=0) -> L.Desc:
+              in_order=L.ORDER_NATURAL, workspace_limit=0, split_hint=0, mid_fmt=None) -> L.Desc:
     lib = L.load()
     d = L.Desc()
     L.check(lib.mfft_desc_default(ctypes.byref(d)), "mfft_desc_default")
@@ -49,6 +49,7 @@ def make_desc(precision="split_fp16", direction=L.FORWARD, scale=None, in_fmt=No
     d.in_order = int(in_order)
     d.workspace_limit = int(workspace_limit)
     d.split_hint = int(split_hint)
+    d.mid_fmt = -1 if mid_fmt is None else _enum(L.FORMATS, mid_fmt)
     return d
 
 

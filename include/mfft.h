@@ -64,6 +64,9 @@ typedef struct mfft_desc {
     int in_order;             /* MFFT_ORDER_* */
     int64_t workspace_limit;  /* bytes of intermediate buffer per plan (0 = default) */
     int split_hint;           /* first-pass length for multi-pass 1D (0 = planner/autotuner) */
+    int mid_fmt;              /* format of intermediates between passes: -1 = the precision's
+                                 default (fp16 -> C16, bf16 -> CBF16, split/fp32 -> C32), or an
+                                 MFFT_FMT_*; split fp16 with C16 rounds once per pass boundary */
 } mfft_desc;
 
 typedef struct mfft_plan_s *mfft_plan;
@@ -119,6 +122,17 @@ int mfft_exec_events(mfft_plan plan, const void *in, void *out, void *stream, vo
  * position i of a digit-reversed buffer holds natural element rev[i]. For 2D plans,
  * axis 0 = x (rev_x), axis 1 = y (rev_y). */
 int mfft_input_permutation(mfft_plan plan, int axis, int *rev, int n);
+
+/* On-device staging (replaces the reference's host-side setup()/finalize() loops,
+ * tcfft_half.cu:1385-1405, and its rev / half2double kernels,
+ * tensorcore_fft_conv2d_kernel.cu:1277-1285, 1407-1414): converts `batch` blocks of
+ * n_inner complex elements from src_fmt to dst_fmt (MFFT_FMT_* or MFFT_FMT_C64 for
+ * interleaved fp64), multiplying by `scale`, and gathers dst[b][i] = src[b][perm_x[i]]
+ * (1D) or dst[b][x][y] = src[b][perm_x[x]][perm_y[y]] (2D, n_inner = nx * ny) when the
+ * device tables are given (NULL = identity). FP64 values are rounded once. */
+#define MFFT_FMT_C64 3
+int mfft_stage_convert(const void *src, int src_fmt, void *dst, int dst_fmt, int64_t n_inner, int64_t batch,
+                       const int *perm_x, int nx, const int *perm_y, int ny, double scale, void *stream);
 
 /* Message of the calling thread's last error ("" if none). */
 const char *mfft_last_error(void);

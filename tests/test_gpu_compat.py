@@ -1,0 +1,139 @@
+"""GPU parity of the reference-named API (include/tcfft_half.h, tcfft_half_2d.h) and the
+on-device staging kernel, called through the C ABI.
+
+Reference contract (tcfft_half.cu:1177-1233, 1376-1413): tcfftExec transforms device
+half2 data in place, input digit-reversed with gen_rev(plan.radices), output natural
+order, unnormalised forward DFT. The precision here is fp16 matrix-core operands with
+fp32 accumulation; tolerance rel-L2 <= 4e-3, rel-max <= 2e-2 vs the FP64 oracle on the
+fp16-rounded input.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import complex_input, from_device, rel_errors, to_device
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _lib():
+    from fft_demo_amd import _lib as L
+
+    return L, L.load()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,batch", [(256, 4096), (512, 64), (1024, 32), (4096, 8), (65536, 4), (131072, 2)])
+def test_tcfft_exec_digit_reversed_in_place(dev, n, batch):
+    import torch
+
+    L, lib = _lib()
+    h = L.TcfftHandle()
+    lib.tcfftCreate(ctypes.byref(h), n, batch)
+    assert lib.tcfftGetLastError() == 0, lib.mfft_last_error()
+    radices = list(h.radices[: h.n_radices])
+    assert np.prod(radices) == n
+    x = complex_input((batch, n), 100 + n % 97)
+    rev = oracle.gen_rev(n, radices)
+    buf = to_device(x[:, rev], torch.float16, dev)  # setup()'s host permutation
+    lib.tcfftExec(h, ctypes.c_void_p(buf.data_ptr()))
+    assert lib.tcfftGetLastError() == 0
+    torch.cuda.synchronize()
+    y = from_device(buf)
+    ref = oracle.fft(x)
+    l2, mx = rel_errors(y, ref)
+    print(f"tcfftExec n={n} b={batch} radices={radices} passes={h.n_mergings} rel_l2={l2:.2e} max={mx:.2e}")
+    assert l2 <= 4e-3 and mx <= 2e-2
+    lib.tcfftDestroy(ctypes.byref(h))
+
+
+@pytest.mark.gpu
+def test_harness_setup_doit_finalize(dev):
+    """main.cpp:164-180 shape: natural-order FP64 host data in and out."""
+    L, lib = _lib()
+    n, b = 256, 8
+    v = np.floor(oracle.uniform(2 * n * b, 42, 0, 10)) * 0.1  # the reference's 0.1*(rand()%10) values
+    data = np.ascontiguousarray(v)
+    res = np.zeros_like(data)
+    dp = data.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    lib.setup(dp, n, b)
+    lib.doit(1)
+    lib.finalize(res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), b)
+    assert lib.tcfftGetLastError() == 0
+    x = oracle.round_fp16(data).reshape(b, n, 2)
+    ref = oracle.fft(x[..., 0] + 1j * x[..., 1])
+    r = res.reshape(b, n, 2)
+    l2, mx = rel_errors(r[..., 0] + 1j * r[..., 1], ref)
+    capped = oracle.err_capped(r[..., 0] + 1j * r[..., 1], ref)
+    print(f"harness rel_l2={l2:.2e} capped(main.cpp:118)={capped:.2e}")
+    assert l2 <= 4e-3
+    # comparator harness (plain fp32)
+    lib.setup_cu(dp, n, b)
+    lib.doit_cu(1)
+    lib.finalize_cu(res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, b)
+    r = res.reshape(b, n, 2)
+    ref32 = oracle.fft(data.reshape(b, n, 2)[..., 0] + 1j * data.reshape(b, n, 2)[..., 1])
+    l2, _ = rel_errors(r[..., 0] + 1j * r[..., 1], ref32)
+    assert l2 <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(256, 256), (512, 256), (256, 1024), (1024, 1024)])
+def test_tcfft_exec_2d_digit_reversed(dev, nx, ny):
+    import torch
+
+    L, lib = _lib()
+    h = L.TcfftHandle2d()
+    b = 2
+    lib.tcfftCreate_2d(ctypes.byref(h), nx, ny, b)
+    assert lib.tcfftGetLastError() == 0, lib.mfft_last_error()
+    rx = oracle.gen_rev(nx, list(h.radices_x[: h.n_radices_x]))
+    ry = oracle.gen_rev(ny, list(h.radices_y[: h.n_radices_y]))
+    x = complex_input((b, nx, ny), 7)
+    buf = to_device(x[:, rx][:, :, ry], torch.float16, dev)  # setup_2d (tcfft_half_2d.cu:1264-1271)
+    lib.tcfftExec_2d(h, ctypes.c_void_p(buf.data_ptr()))
+    torch.cuda.synchronize()
+    assert lib.tcfftGetLastError() == 0
+    y = from_device(buf)
+    ref = oracle.fft2d(x)
+    l2, mx = rel_errors(y, ref)
+    print(f"tcfftExec_2d {nx}x{ny} rel_l2={l2:.2e} max={mx:.2e}")
+    assert l2 <= 4e-3 and mx <= 2e-2
+    lib.tcfftDestroy_2d(ctypes.byref(h))
+
+
+@pytest.mark.gpu
+def test_stage_convert_rounding_and_permutation(dev):
+    import torch
+
+    L, lib = _lib()
+    n, b = 512, 3
+    v = np.concatenate([oracle.uniform(2 * n * b - 6, 5, -70000, 70000), [65519.9, 65520.0, 1e-8, -3e-5, 2.0 ** -25, 0.0]])
+    src = torch.from_numpy(v).to(dev)
+    dst = torch.empty(2 * n * b, dtype=torch.float16, device=dev)
+    rev = oracle.gen_rev(n, [16, 16, 2])
+    perm = torch.from_numpy(rev.astype(np.int32)).to(dev)
+    rc = lib.mfft_stage_convert(ctypes.c_void_p(src.data_ptr()), 3, ctypes.c_void_p(dst.data_ptr()), 0, n, b,
+                                ctypes.c_void_p(perm.data_ptr()), n, None, 0, 1.0, None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = dst.double().cpu().numpy().reshape(b, n, 2)
+    want = oracle.round_fp16(v).reshape(b, n, 2)[:, rev]
+    assert np.array_equal(np.nan_to_num(got, posinf=1e9, neginf=-1e9), np.nan_to_num(want, posinf=1e9, neginf=-1e9))
+
+
+@pytest.mark.gpu
+def test_compat_cpp_program():
+    """A C++ caller of the reference-named API, built against include/ and the library."""
+    exe = os.path.join(ROOT, "fft_demo_amd", "lib", "compat_main")
+    assert os.path.exists(exe), "build the library first"
+    out = subprocess.run([exe, "512", "4", "32", "64"], check=True, capture_output=True, text=True, timeout=120).stdout
+    print(out)
+    for line in out.strip().splitlines():
+        fields = dict(kv.split("=") for kv in line.split()[1:] if "=" in kv)
+        assert fields["err"] == "0"
+        assert float(fields["rel_l2"]) <= 4e-3, line
