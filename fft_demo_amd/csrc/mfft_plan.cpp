@@ -29,6 +29,7 @@ typedef void (*tile_fn)(TileArgs);
 tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
 int pow2_tile_lines(int R, int TL);
 hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream);
+const void *find_small_kernel(int R, int inf, int outf);
 hipError_t stage_convert(const void *src, int sfmt, void *dst, int dfmt, long long n_inner, long long batch,
                          const int *perm_x, int nx, const int *perm_y, int ny, double scale, hipStream_t s);
 }  // namespace mfft
@@ -129,6 +130,7 @@ struct PassSpec {
     std::vector<int> radix;
     bool generic = false;
     int fast_kind = -1;   // KIND_* of the fast kernel, -1 = tile kernel
+    bool small = false;   // streaming short-DFT kernel (R <= 16, one thread per line)
     float2 *stage_tw_s = nullptr, *tw_lo_s = nullptr;
     float last_mul = 1.f, tail_mul = 1.f;
     long long max_grid = 0;  // persistent grid of the fast kernel
@@ -206,6 +208,10 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     long long L1 = 0;
     if (split_hint > 1 && L % split_hint == 0 && split_hint <= 256 && factorable(split_hint)) {
         L1 = split_hint;
+    } else if (is_pow2(L) && ps_in != 1 && L % 256 == 0 && L / 256 <= 256) {
+        // strided axis (2D columns): both passes are strided; give the second the
+        // 256-point matrix-core kernel and the small remainder to the first
+        L1 = L / 256;
     } else if (is_pow2(L)) {
         int m = 0;
         while ((1LL << m) < L) ++m;
@@ -277,8 +283,22 @@ static int finish_passes(mfft_plan_s *p)
             pow2 = false;
         }
         ps.generic = !pow2;
+        // short strided DFTs: streaming kernel, one thread per line
+        if (ps.perm_axis < 0 && ps.ps_in != 1 && (ps.R == 2 || ps.R == 4 || ps.R == 8 || ps.R == 16)) {
+            int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
+            int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+            const void *fn = find_small_kernel(ps.R, in_fmt, out_fmt);
+            if (fn) {
+                ps.small = true;
+                ps.fn = fn;
+                ps.l0 = unit_in >= 0 ? unit_in : 0;
+                ps.T = 1;
+                ps.TL = 1;
+                ps.radix.assign(1, ps.R);
+            }
+        }
         // fast kernel: power of two 256..4096, no fused permutation
-        if (pow2 && ps.R >= 256 && ps.R <= kMaxTile && ps.perm_axis < 0) {
+        if (!ps.small && pow2 && ps.R >= 256 && ps.R <= kMaxTile && ps.perm_axis < 0) {
             int kind = -1, l0 = -1;
             int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
             int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
@@ -319,7 +339,7 @@ static int finish_passes(mfft_plan_s *p)
                 ps.radix = r2;
             }
         }
-        if (ps.fast_kind >= 0) {
+        if (ps.fast_kind >= 0 || ps.small) {
             // l0 chosen above
         } else if (!ps.generic) {
             ps.T = pow2_tile_lines(ps.R, ps.TL);
@@ -340,7 +360,7 @@ static int finish_passes(mfft_plan_s *p)
         ps.l2 = rest.size() > 1 ? rest[1] : -1;
         if (ps.tw_N && ps.tw_level != ps.l0 && ps.tw_level != ps.l1)
             return fail(MFFT_ERR_UNSUPPORTED, "twiddle level layout");
-        if (ps.fast_kind < 0) {
+        if (ps.fast_kind < 0 && !ps.small) {
             ps.fn = (const void *)find_tile_kernel(p->d.precision, ps.R, ps.TL, ps.generic);
             if (!ps.fn) return fail(MFFT_ERR_UNSUPPORTED, "no kernel for R=%d TL=%d", ps.R, ps.TL);
         }
@@ -391,6 +411,7 @@ static int build_tables(mfft_plan_s *p)
                 HIP_TRY(gen_twiddle(ps.tw_lo_s, ps.tw_N, nlo, 1, sign, 0, fac));
             }
         }
+        if (ps.small) ps.last_mul = p->d.scale == MFFT_SCALE_STAGE ? 1.0f / ps.R : 1.0f;
         if (ps.perm_axis >= 0 && p->d.in_order == MFFT_ORDER_DIGIT_REVERSED) {
             const std::vector<int> &rev = p->rev_axis[ps.perm_axis];
             if ((int)rev.size() != ps.R) return fail(MFFT_ERR_INVALID, "permutation size");
@@ -504,6 +525,12 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         if (ntiles <= 0) continue;
         if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
         if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
+        if (ps.small) {
+            long long nl = (long long)a.c0 * a.c1 * a.c2;
+            void *args[] = {&a};
+            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)((nl + 255) / 256)), dim3(256), args, 0, s));
+            continue;
+        }
         if (ps.fast_kind >= 0) {
             a.tw_lds = ps.tw_N && (1 << ps.tw_bits) <= kFastTw && ((ps.tw_N + (1 << ps.tw_bits) - 1) >> ps.tw_bits) <= kFastTw;
             long long grid = std::min(ntiles, ps.max_grid);

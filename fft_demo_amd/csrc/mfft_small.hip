@@ -1,0 +1,148 @@
+// Streaming kernel for short strided DFTs (R = 2, 4, 8, 16 points per line) -- the small
+// first pass of a four-step split along a strided axis (e.g. 1024 = 4 x 256 columns of a
+// 2D transform). One thread owns one line: it loads the R points (consecutive threads take
+// consecutive lines, so every load is coalesced), transforms them in fp32 registers,
+// applies the four-step twiddle and the stage scale, and stores -- no LDS, one HBM pass.
+#include "mfft_common.h"
+
+namespace mfft {
+namespace small {
+
+template <int FMT>
+__device__ __forceinline__ float2 ld(const void *p, long long i)
+{
+    if constexpr (FMT == FMT_C32) {
+        return reinterpret_cast<const float2 *>(p)[i];
+    } else {
+        uint32_t w = reinterpret_cast<const uint32_t *>(p)[i];
+        if constexpr (FMT == FMT_C16)
+            return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffff)),
+                               (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
+        else
+            return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+    }
+}
+
+template <int FMT>
+__device__ __forceinline__ void st(void *p, long long i, float2 v)
+{
+    if constexpr (FMT == FMT_C32) {
+        reinterpret_cast<float2 *>(p)[i] = v;
+    } else if constexpr (FMT == FMT_C16) {
+        reinterpret_cast<uint32_t *>(p)[i] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.x) |
+                                             ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.y) << 16);
+    } else {
+        reinterpret_cast<uint32_t *>(p)[i] = (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.x) |
+                                             ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.y) << 16);
+    }
+}
+
+__device__ __forceinline__ float2 cm(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ void dft4(float2 *v, int sg)
+{
+    float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y), d0 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    float2 s1 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y), d1 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    float2 r = make_float2(-sg * d1.y, sg * d1.x);
+    v[0] = make_float2(s0.x + s1.x, s0.y + s1.y);
+    v[1] = make_float2(d0.x + r.x, d0.y + r.y);
+    v[2] = make_float2(s0.x - s1.x, s0.y - s1.y);
+    v[3] = make_float2(d0.x - r.x, d0.y - r.y);
+}
+
+// DFT of R points in place, fp32, exact small-angle constants for W_R.
+template <int R>
+__device__ __forceinline__ void dft(float2 *v, int sg)
+{
+    if constexpr (R == 2) {
+        float2 a = v[0], b = v[1];
+        v[0] = make_float2(a.x + b.x, a.y + b.y);
+        v[1] = make_float2(a.x - b.x, a.y - b.y);
+    } else if constexpr (R == 4) {
+        dft4(v, sg);
+    } else {
+        // R = 2 * H (H = 4 or 8): radix-2 butterfly, W_R^j on the odd half, two DFT_H
+        constexpr int H = R / 2;
+        float2 a[H], b[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            a[j] = make_float2(v[j].x + v[j + H].x, v[j].y + v[j + H].y);
+            b[j] = make_float2(v[j].x - v[j + H].x, v[j].y - v[j + H].y);
+        }
+        const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+        // W_16^e (cos, sin) for e = 0..7; W_8^e = W_16^(2e)
+        const float wc[8] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1};
+        const float ws[8] = {0.f, s1, h, c1, 1.f, c1, h, s1};
+#pragma unroll
+        for (int j = 1; j < H; ++j) {
+            const int e = j * (16 / R);
+            b[j] = cm(b[j], make_float2(wc[e], sg * ws[e]));
+        }
+        dft<H>(a, sg);
+        dft<H>(b, sg);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            v[2 * k] = a[k];
+            v[2 * k + 1] = b[k];
+        }
+    }
+}
+
+// Lines are (l0, l1, l2) as in TileArgs; one thread per line.
+template <int R, int INF, int OUTF>
+__global__ void __launch_bounds__(256) small_cols(TileArgs a)
+{
+    const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long nl = (long long)a.c0 * a.c1 * a.c2;
+    if (id >= nl) return;
+    const int l0 = (int)(id % a.c0);
+    const long long r = id / a.c0;
+    const int l1 = (int)(r % a.c1);
+    const long long l2 = r / a.c1;
+    const long long ib = (long long)l0 * a.in_s0 + (long long)l1 * a.in_s1 + l2 * a.in_s2;
+    const long long ob = (long long)l0 * a.out_s0 + (long long)l1 * a.out_s1 + l2 * a.out_s2;
+    float2 v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = ld<INF>(a.in, ib + (long long)j * a.in_ps);
+    dft<R>(v, a.sign);
+    const unsigned mult = (unsigned)(a.tw_level ? l1 : l0);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        float2 y = make_float2(v[k].x * a.last_mul, v[k].y * a.last_mul);
+        if (a.tw_N && k) {
+            const unsigned e = mult * (unsigned)k;
+            y = cm(y, cm(a.tw_hi[e >> a.tw_bits], a.tw_lo[e & ((1u << a.tw_bits) - 1u)]));
+        }
+        st<OUTF>(a.out, ob + (long long)k * a.out_ps, y);
+    }
+}
+
+template <int INF, int OUTF>
+const void *pick(int R)
+{
+    switch (R) {
+    case 2: return (const void *)small_cols<2, INF, OUTF>;
+    case 4: return (const void *)small_cols<4, INF, OUTF>;
+    case 8: return (const void *)small_cols<8, INF, OUTF>;
+    case 16: return (const void *)small_cols<16, INF, OUTF>;
+    default: return nullptr;
+    }
+}
+
+}  // namespace small
+
+const void *find_small_kernel(int R, int inf, int outf)
+{
+    using namespace small;
+    if (inf == FMT_C16 && outf == FMT_C16) return pick<FMT_C16, FMT_C16>(R);
+    if (inf == FMT_C16 && outf == FMT_C32) return pick<FMT_C16, FMT_C32>(R);
+    if (inf == FMT_C32 && outf == FMT_C32) return pick<FMT_C32, FMT_C32>(R);
+    if (inf == FMT_C32 && outf == FMT_C16) return pick<FMT_C32, FMT_C16>(R);
+    if (inf == FMT_CBF16 && outf == FMT_CBF16) return pick<FMT_CBF16, FMT_CBF16>(R);
+    return nullptr;
+}
+
+}  // namespace mfft
