@@ -125,6 +125,32 @@ SIGNATURES = {
 }
 
 
+class ConvDesc(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_int), ("channels", ctypes.c_int), ("filters", ctypes.c_int), ("h", ctypes.c_int),
+                ("w", ctypes.c_int), ("fh", ctypes.c_int), ("fw", ctypes.c_int), ("mode", ctypes.c_int),
+                ("pad_h", ctypes.c_int), ("pad_w", ctypes.c_int), ("precision", ctypes.c_int),
+                ("workspace_limit", ctypes.c_int64)]
+
+
+class ConvInfo(ctypes.Structure):
+    _fields_ = [("pad_h", ctypes.c_int), ("pad_w", ctypes.c_int), ("out_h", ctypes.c_int), ("out_w", ctypes.c_int),
+                ("device_bytes", ctypes.c_int64), ("spectrum_bytes", ctypes.c_int64)]
+
+
+CONV_SAME, CONV_VALID_CORR = 0, 1
+REAL_F32, REAL_F64 = 0, 1
+
+SIGNATURES.update({
+    "mfft_conv2d_desc_default": (ctypes.c_int, [ctypes.POINTER(ConvDesc)]),
+    "mfft_conv2d_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(ConvDesc)]),
+    "mfft_conv2d_set_filters": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "mfft_conv2d_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "mfft_conv2d_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(ConvInfo)]),
+    "mfft_conv2d_destroy": (ctypes.c_int, [_vp]),
+    "mfft_good_size": (ctypes.c_int, [ctypes.c_int]),
+})
+
+
 class TcfftHandle(ctypes.Structure):
     """struct tcfftHandle of include/tcfft_half.h (reference tcfft_half.h:7-18)."""
     _fields_ = [("N", ctypes.c_int), ("N_batch", ctypes.c_int), ("radices", ctypes.c_int * 9),

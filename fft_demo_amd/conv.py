@@ -1,0 +1,84 @@
+"""FFT-based 2D convolution operator: Python handle over mfft_conv2d_* (include/mfft.h).
+
+Conv2d(batch, channels, filters, h, w, fh, fw, mode="same"|"valid_corr") computes
+out[b][f] = sum_c in[b][c] (*) filt[f][c] in the HIP kernels of libmfft.so. Filter spectra
+are computed once by set_filters() and stay resident on the device.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib as L
+
+MODES = {"same": L.CONV_SAME, "valid_corr": L.CONV_VALID_CORR}
+
+
+class Conv2d:
+    def __init__(self, batch, channels, filters, h, w, fh, fw, mode="same", pad=None, precision="split_fp16",
+                 workspace_limit=0):
+        lib = L.load()
+        d = L.ConvDesc()
+        lib.mfft_conv2d_desc_default(ctypes.byref(d))
+        d.batch, d.channels, d.filters, d.h, d.w, d.fh, d.fw = batch, channels, filters, h, w, fh, fw
+        d.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+        if pad is not None:
+            d.pad_h, d.pad_w = (pad, pad) if isinstance(pad, int) else pad
+        d.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        d.workspace_limit = workspace_limit
+        h_ = ctypes.c_void_p()
+        L.check(lib.mfft_conv2d_create(ctypes.byref(h_), ctypes.byref(d)), "mfft_conv2d_create")
+        self._h = h_
+        self.desc = d
+        inf = L.ConvInfo()
+        L.check(lib.mfft_conv2d_info_get(self._h, ctypes.byref(inf)), "mfft_conv2d_info_get")
+        self.pad = (inf.pad_h, inf.pad_w)
+        self.out_shape = (batch, filters, inf.out_h, inf.out_w)
+        self.device_bytes = inf.device_bytes
+        self.spectrum_bytes = inf.spectrum_bytes
+
+    @staticmethod
+    def _fmt(t):
+        import torch
+
+        if t.dtype == torch.float32:
+            return L.REAL_F32
+        if t.dtype == torch.float64:
+            return L.REAL_F64
+        raise TypeError(f"unsupported dtype {t.dtype}")
+
+    def set_filters(self, filt, stream=None):
+        import torch
+
+        d = self.desc
+        assert filt.is_cuda and filt.is_contiguous() and filt.numel() == d.filters * d.channels * d.fh * d.fw
+        s = stream if stream is not None else torch.cuda.current_stream(filt.device)
+        L.check(L.load().mfft_conv2d_set_filters(self._h, ctypes.c_void_p(filt.data_ptr()), self._fmt(filt),
+                                                  ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_set_filters")
+
+    def forward(self, x, out=None, stream=None):
+        import torch
+
+        d = self.desc
+        assert x.is_cuda and x.is_contiguous() and x.numel() == d.batch * d.channels * d.h * d.w
+        if out is None:
+            out = torch.empty(self.out_shape, dtype=x.dtype, device=x.device)
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        L.check(L.load().mfft_conv2d_forward(self._h, ctypes.c_void_p(x.data_ptr()), self._fmt(x),
+                                              ctypes.c_void_p(out.data_ptr()), self._fmt(out),
+                                              ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_forward")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.load().mfft_conv2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def good_size(n: int) -> int:
+    return int(L.load().mfft_good_size(int(n)))

@@ -1,0 +1,145 @@
+// Elementwise kernels of the FFT convolution operator (pack/pad, pointwise multiply-
+// accumulate, crop/unpack). The transforms themselves are mfft 2D plans.
+//
+// Reference steps these replace (tensorcore_fft_conv2d_kernel.cu): filter rotation
+// (fft_conv2d_inverse_filter :1234-1253), real->complex padding with digit reversal
+// (fft_conv2d_bit_rev_real_pad_to_cpx_double2half :1222-1232), the channel sum
+// (fft_cpx_point_wise_mul_add :1033-1056), the per-filter power-of-10 quantisation
+// (quant_p1/p2/p3 :1058-1098) -- not needed here: spectra stay fp32 and the forward
+// transform is scaled per stage -- and the crop/scale readout
+// (read_out_scale_double2double_kernel :1390-1397).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfft {
+
+// Two real images (batch items 2k and 2k+1, same channel) become the real and imaginary
+// parts of one complex image, zero-padded to Ph x Pw at the origin.
+// src: real [B][C][h][w] (fp32 or fp64); dst: complex fp32 [pairs][C][Ph][Pw].
+template <typename T>
+__global__ void pack_pairs_kernel(const T *src, float2 *dst, int B, int C, int h, int w, int Ph, int Pw, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % Pw);
+    long long r = i / Pw;
+    int y = (int)(r % Ph);
+    r /= Ph;
+    int c = (int)(r % C);
+    long long k = r / C;
+    float2 v = make_float2(0.f, 0.f);
+    if (y < h && x < w) {
+        long long b0 = 2 * k, b1 = 2 * k + 1;
+        long long off = ((long long)c * h + y) * w + x;
+        v.x = (float)src[b0 * C * h * w + off];
+        if (b1 < B) v.y = (float)src[b1 * C * h * w + off];
+    }
+    dst[i] = v;
+}
+
+// Filter [F][C][fh][fw] real -> complex fp32 [F][C][Ph][Pw], optionally rotated by 180
+// degrees (convolution by the rotated filter = cross-correlation).
+template <typename T>
+__global__ void pack_filters_kernel(const T *src, float2 *dst, int fh, int fw, int Ph, int Pw, int rotate, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % Pw);
+    long long r = i / Pw;
+    int y = (int)(r % Ph);
+    long long fc = r / Ph;
+    float2 v = make_float2(0.f, 0.f);
+    if (y < fh && x < fw) {
+        int sy = rotate ? fh - 1 - y : y, sx = rotate ? fw - 1 - x : x;
+        v.x = (float)src[(fc * fh + sy) * fw + sx];
+    }
+    dst[i] = v;
+}
+
+// Y[k][f][p] = sum_c X[k][c][p] * H[f][c][p]
+__global__ void pointwise_mac_kernel(const float2 *X, const float2 *H, float2 *Y, int C, int F, long long P,
+                                     long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    long long p = i % P;
+    long long r = i / P;
+    int f = (int)(r % F);
+    long long k = r / F;
+    float re = 0.f, im = 0.f;
+    for (int c = 0; c < C; ++c) {
+        float2 a = X[(k * C + c) * P + p];
+        float2 b = H[((long long)f * C + c) * P + p];
+        re += a.x * b.x - a.y * b.y;
+        im += a.x * b.y + a.y * b.x;
+    }
+    Y[i] = make_float2(re, im);
+}
+
+// out[b][f][y][x] = scale * (Re or Im) Y[b/2][f][y + oy][x + ox]
+template <typename T>
+__global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
+                                   int ox, float scale, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % ow);
+    long long r = i / ow;
+    int y = (int)(r % oh);
+    r /= oh;
+    int f = (int)(r % F);
+    long long b = r / F;
+    float2 v = Y[(((b >> 1) * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox)];
+    out[i] = (T)((b & 1 ? v.y : v.x) * scale);
+}
+
+static inline unsigned nblocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t conv_pack_pairs(const void *src, int f64, float2 *dst, int B, int C, int h, int w, int Ph, int Pw,
+                           hipStream_t s)
+{
+    long long pairs = (B + 1) / 2, total = pairs * C * (long long)Ph * Pw;
+    if (f64)
+        hipLaunchKernelGGL(pack_pairs_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, (const double *)src, dst,
+                           B, C, h, w, Ph, Pw, total);
+    else
+        hipLaunchKernelGGL(pack_pairs_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, (const float *)src, dst,
+                           B, C, h, w, Ph, Pw, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_pack_filters(const void *src, int f64, float2 *dst, int FC, int fh, int fw, int Ph, int Pw, int rotate,
+                             hipStream_t s)
+{
+    long long total = (long long)FC * Ph * Pw;
+    if (f64)
+        hipLaunchKernelGGL(pack_filters_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, (const double *)src, dst,
+                           fh, fw, Ph, Pw, rotate, total);
+    else
+        hipLaunchKernelGGL(pack_filters_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, (const float *)src, dst,
+                           fh, fw, Ph, Pw, rotate, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long pairs, int C, int F, long long P,
+                          hipStream_t s)
+{
+    long long total = pairs * F * P;
+    hipLaunchKernelGGL(pointwise_mac_kernel, dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C, F, P, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
+                     float scale, hipStream_t s)
+{
+    long long total = (long long)B * F * oh * ow;
+    if (f64)
+        hipLaunchKernelGGL(crop_unpack_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out, B, F,
+                           oh, ow, Ph, Pw, oy, ox, scale, total);
+    else
+        hipLaunchKernelGGL(crop_unpack_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out, B, F, oh,
+                           ow, Ph, Pw, oy, ox, scale, total);
+    return hipGetLastError();
+}
+
+}  // namespace mfft

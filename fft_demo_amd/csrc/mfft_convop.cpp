@@ -1,0 +1,235 @@
+// FFT-based 2D convolution operator (C ABI in include/mfft.h).
+//
+// Reference operator: class fft_conv2d_half (tensorcore_fft_conv2d.h:41-197); forward
+// path = ctor (tensorcore_fft_conv2d.cpp:12-58: pad, digit-reverse, FFT signal and
+// filter), TENSORCORE_FFT_CONV_2D (:250-273: pointwise, inverse FFT, dequantise) and
+// readout_fwd (:275-282: crop at fx-1, scale 1/(Nx*Ny)). Differences by design: any image
+// size (padding to 2^a*{1,3,5,17} instead of 256/512 only, :23-32), 'same' convolution
+// besides the reference's 'valid' correlation, batches with pairs of real images sharing
+// a complex transform, fp32 spectra with per-stage-scaled split-fp16 transforms instead of
+// fp16 spectra with per-filter power-of-10 quantisation, resident filter spectra reused
+// across calls, and no per-call allocation.
+#include "../../include/mfft.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace mfft {
+hipError_t conv_pack_pairs(const void *src, int f64, float2 *dst, int B, int C, int h, int w, int Ph, int Pw,
+                           hipStream_t s);
+hipError_t conv_pack_filters(const void *src, int f64, float2 *dst, int FC, int fh, int fw, int Ph, int Pw, int rotate,
+                             hipStream_t s);
+hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long pairs, int C, int F, long long P,
+                          hipStream_t s);
+hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
+                     float scale, hipStream_t s);
+}  // namespace mfft
+
+struct mfft_conv2d_s {
+    mfft_conv2d_desc d{};
+    int Ph = 0, Pw = 0, oh = 0, ow = 0, oy = 0, ox = 0;
+    long long pairs = 0;
+    mfft_plan fwd_img = nullptr, fwd_filt = nullptr, inv = nullptr;
+    float2 *H = nullptr, *X = nullptr, *Y = nullptr;
+    bool have_filters = false;
+    long long bytes = 0, spec_bytes = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+thread_local std::string c_err;
+
+int cfail(int code, const char *fmt, ...)
+{
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c_err = buf;
+    return code;
+}
+
+bool good(int n)
+{
+    if (n < 2 || n > 4096 * 256) return false;
+    while (n % 2 == 0) n /= 2;
+    const int odd[] = {17, 5, 3};
+    for (int p : odd)
+        if (n % p == 0) n /= p;  // at most one odd factor keeps the schedule short
+    return n == 1;
+}
+
+int plan_info_bytes(mfft_plan p)
+{
+    mfft_info inf;
+    if (mfft_plan_info(p, &inf)) return 0;
+    return (int)(inf.workspace_bytes >> 10);  // KiB
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfft_good_size(int n)
+{
+    for (int m = n < 2 ? 2 : n; m <= 4096 * 256; ++m)
+        if (good(m)) {
+            // keep sizes the transform handles in at most two passes per axis
+            return m;
+        }
+    return -1;
+}
+
+int mfft_conv2d_desc_default(mfft_conv2d_desc *d)
+{
+    if (!d) return MFFT_ERR_INVALID;
+    memset(d, 0, sizeof *d);
+    d->batch = d->channels = d->filters = 1;
+    d->mode = MFFT_CONV_SAME;
+    d->precision = MFFT_PREC_SPLIT_FP16;
+    return MFFT_OK;
+}
+
+int mfft_conv2d_destroy(mfft_conv2d op)
+{
+    if (!op) return MFFT_OK;
+    (void)hipDeviceSynchronize();
+    if (op->fwd_img) mfft_plan_destroy(op->fwd_img);
+    if (op->fwd_filt) mfft_plan_destroy(op->fwd_filt);
+    if (op->inv) mfft_plan_destroy(op->inv);
+    (void)hipFree(op->H);
+    (void)hipFree(op->X);
+    (void)hipFree(op->Y);
+    delete op;
+    return MFFT_OK;
+}
+
+int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
+{
+    if (!out || !d) return cfail(MFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->batch < 1 || d->channels < 1 || d->filters < 1 || d->h < 1 || d->w < 1 || d->fh < 1 || d->fw < 1)
+        return cfail(MFFT_ERR_INVALID, "bad sizes");
+    if (d->mode != MFFT_CONV_SAME && d->mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_INVALID, "mode");
+    if (d->mode == MFFT_CONV_VALID_CORR && (d->fh > d->h || d->fw > d->w))
+        return cfail(MFFT_ERR_INVALID, "filter larger than the image in 'valid' mode");
+    mfft_conv2d op = new mfft_conv2d_s;
+    op->d = *d;
+    int need_h = d->h + d->fh - 1, need_w = d->w + d->fw - 1;
+    if (d->mode == MFFT_CONV_VALID_CORR) {
+        // circular wrap-around only touches outputs outside the 'valid' window
+        need_h = d->h;
+        need_w = d->w;
+    }
+    op->Ph = d->pad_h > 0 ? d->pad_h : mfft_good_size(need_h);
+    op->Pw = d->pad_w > 0 ? d->pad_w : mfft_good_size(need_w);
+    if (op->Ph < need_h || op->Pw < need_w) {
+        mfft_conv2d_destroy(op);
+        return cfail(MFFT_ERR_INVALID, "padding %dx%d smaller than %dx%d", op->Ph, op->Pw, need_h, need_w);
+    }
+    if (d->mode == MFFT_CONV_SAME) {
+        op->oh = d->h;
+        op->ow = d->w;
+        op->oy = (d->fh - 1) / 2;
+        op->ox = (d->fw - 1) / 2;
+    } else {
+        op->oh = d->h - d->fh + 1;
+        op->ow = d->w - d->fw + 1;
+        op->oy = d->fh - 1;
+        op->ox = d->fw - 1;
+    }
+    op->pairs = (d->batch + 1) / 2;
+    mfft_desc fd;
+    mfft_desc_default(&fd);
+    fd.precision = d->precision;
+    fd.in_fmt = fd.out_fmt = MFFT_FMT_C32;
+    fd.workspace_limit = d->workspace_limit;
+    fd.scale = MFFT_SCALE_STAGE;  // image spectra carry 1/(Ph*Pw): the inverse is then unit
+    int rc = mfft_plan_create_2d(&op->fwd_img, op->Ph, op->Pw, op->pairs * d->channels, &fd);
+    if (!rc) {
+        fd.scale = MFFT_SCALE_NONE;
+        rc = mfft_plan_create_2d(&op->fwd_filt, op->Ph, op->Pw, (int64_t)d->filters * d->channels, &fd);
+    }
+    if (!rc) {
+        fd.direction = MFFT_INVERSE;
+        rc = mfft_plan_create_2d(&op->inv, op->Ph, op->Pw, op->pairs * d->filters, &fd);
+    }
+    if (rc) {
+        std::string m = mfft_last_error();
+        mfft_conv2d_destroy(op);
+        return cfail(rc, "%s", m.c_str());
+    }
+    long long P = (long long)op->Ph * op->Pw;
+    op->spec_bytes = (long long)d->filters * d->channels * P * 8;
+    long long xb = op->pairs * d->channels * P * 8, yb = op->pairs * d->filters * P * 8;
+    if (hipMalloc(&op->H, op->spec_bytes) != hipSuccess || hipMalloc(&op->X, xb) != hipSuccess ||
+        hipMalloc(&op->Y, yb) != hipSuccess) {
+        mfft_conv2d_destroy(op);
+        return cfail(MFFT_ERR_NOMEM, "conv buffers (%lld + %lld + %lld bytes)", op->spec_bytes, xb, yb);
+    }
+    op->bytes = op->spec_bytes + xb + yb;
+    mfft_info inf;
+    for (mfft_plan p : {op->fwd_img, op->fwd_filt, op->inv})
+        if (!mfft_plan_info(p, &inf)) op->bytes += inf.workspace_bytes;
+    *out = op;
+    return MFFT_OK;
+}
+
+int mfft_conv2d_set_filters(mfft_conv2d op, const void *filters, int real_fmt, void *stream)
+{
+    if (!op || !filters) return cfail(MFFT_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(op->mu);
+    hipStream_t s = (hipStream_t)stream;
+    const mfft_conv2d_desc &d = op->d;
+    if (mfft::conv_pack_filters(filters, real_fmt == MFFT_REAL_F64, op->H, d.filters * d.channels, d.fh, d.fw, op->Ph,
+                                op->Pw, d.mode == MFFT_CONV_VALID_CORR, s) != hipSuccess)
+        return cfail(MFFT_ERR_HIP, "pack filters");
+    int rc = mfft_exec(op->fwd_filt, op->H, op->H, s);
+    if (rc) return cfail(rc, "%s", mfft_last_error());
+    op->have_filters = true;
+    return MFFT_OK;
+}
+
+int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, void *stream)
+{
+    if (!op || !in || !out) return cfail(MFFT_ERR_INVALID, "null argument");
+    if (!op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
+    std::lock_guard<std::mutex> lk(op->mu);
+    hipStream_t s = (hipStream_t)stream;
+    const mfft_conv2d_desc &d = op->d;
+    if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw, s) !=
+        hipSuccess)
+        return cfail(MFFT_ERR_HIP, "pack");
+    int rc = mfft_exec(op->fwd_img, op->X, op->X, s);
+    if (rc) return cfail(rc, "%s", mfft_last_error());
+    if (mfft::conv_pointwise(op->X, op->H, op->Y, op->pairs, d.channels, d.filters, (long long)op->Ph * op->Pw, s) !=
+        hipSuccess)
+        return cfail(MFFT_ERR_HIP, "pointwise");
+    rc = mfft_exec(op->inv, op->Y, op->Y, s);
+    if (rc) return cfail(rc, "%s", mfft_last_error());
+    if (mfft::conv_crop(op->Y, out, out_fmt == MFFT_REAL_F64, d.batch, d.filters, op->oh, op->ow, op->Ph, op->Pw,
+                        op->oy, op->ox, 1.0f, s) != hipSuccess)
+        return cfail(MFFT_ERR_HIP, "crop");
+    return MFFT_OK;
+}
+
+int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info)
+{
+    if (!op || !info) return cfail(MFFT_ERR_INVALID, "null argument");
+    info->pad_h = op->Ph;
+    info->pad_w = op->Pw;
+    info->out_h = op->oh;
+    info->out_w = op->ow;
+    info->device_bytes = op->bytes;
+    info->spectrum_bytes = op->spec_bytes;
+    return MFFT_OK;
+}
+
+}  // extern "C"

@@ -134,6 +134,49 @@ int mfft_input_permutation(mfft_plan plan, int axis, int *rev, int n);
 int mfft_stage_convert(const void *src, int src_fmt, void *dst, int dst_fmt, int64_t n_inner, int64_t batch,
                        const int *perm_x, int nx, const int *perm_y, int ny, double scale, void *stream);
 
+/* ------------------------------------------------------------------------------------
+ * FFT-based 2D convolution operator (replaces class fft_conv2d_half,
+ * tensorcore_fft_conv2d.h:41-197, forward path tensorcore_fft_conv2d.cpp:12-58, 250-363).
+ * out[b][f] = sum_c in[b][c] (*) filt[f][c] for a batch of B images with C channels and
+ * F filters, via zero-padded 2D FFTs: the filter spectra are computed once
+ * (mfft_conv2d_set_filters) and stay resident; two real images share one complex
+ * transform (real and imaginary parts).
+ *   MFFT_CONV_SAME        'same' convolution: output h x w, centred on the full
+ *                         convolution at offset ((fh-1)/2, (fw-1)/2), zero boundary
+ *   MFFT_CONV_VALID_CORR  the reference's 'valid' cross-correlation: output
+ *                         (h-fh+1) x (w-fw+1) (main.cpp:97-116)
+ * ------------------------------------------------------------------------------------ */
+enum { MFFT_CONV_SAME = 0, MFFT_CONV_VALID_CORR = 1 };
+enum { MFFT_REAL_F32 = 0, MFFT_REAL_F64 = 1 };
+
+typedef struct mfft_conv2d_desc {
+    int batch, channels, filters;
+    int h, w, fh, fw;
+    int mode;           /* MFFT_CONV_* */
+    int pad_h, pad_w;   /* transform size; 0 = smallest supported size >= h + fh - 1 */
+    int precision;      /* MFFT_PREC_* of the transforms (default split fp16) */
+    int64_t workspace_limit;
+} mfft_conv2d_desc;
+
+typedef struct mfft_conv2d_s *mfft_conv2d;
+
+typedef struct mfft_conv2d_info {
+    int pad_h, pad_w, out_h, out_w;
+    int64_t device_bytes;       /* resident spectra + buffers + plan workspaces */
+    int64_t spectrum_bytes;     /* the resident filter spectra alone */
+} mfft_conv2d_info;
+
+int mfft_conv2d_desc_default(mfft_conv2d_desc *d);
+int mfft_conv2d_create(mfft_conv2d *op, const mfft_conv2d_desc *d);
+/* filters: device array [F][C][fh][fw] of MFFT_REAL_*; transforms and keeps the spectra */
+int mfft_conv2d_set_filters(mfft_conv2d op, const void *filters, int real_fmt, void *stream);
+/* in: device [B][C][h][w], out: device [B][F][out_h][out_w] (MFFT_REAL_* each) */
+int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, void *stream);
+int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info);
+int mfft_conv2d_destroy(mfft_conv2d op);
+/* smallest transform length >= n that the planner supports well (2^a * {1,3,5,17}) */
+int mfft_good_size(int n);
+
 /* Message of the calling thread's last error ("" if none). */
 const char *mfft_last_error(void);
 

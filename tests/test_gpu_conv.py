@@ -1,0 +1,89 @@
+"""GPU parity of the FFT-convolution operator against the direct FP64 convolution oracle
+(oracle.conv2d_same: 'same' convolution; oracle.corr2d_valid: the reference's 'valid'
+cross-correlation, main.cpp:97-116).
+
+Tolerance: rel-L2 <= 1e-4 and rel-max <= 1e-3 of each output map (split-fp16 transforms,
+fp32 spectra, fp32 images), measured on every pixel for small images and on sampled
+rows (top, middle, bottom bands) for 2048 x 2048.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def _filters(F, C, k, seed):
+    return oracle.uniform(F * C * k * k, seed, 0.0, 1.0).reshape(F, C, k, k)
+
+
+def _conv(dev, B, C, F, h, w, fh, fw, mode, seed, pad=None, dtype="float32"):
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    img = oracle.uniform(B * C * h * w, seed, 0.0, 1.0).reshape(B, C, h, w)
+    filt = oracle.uniform(F * C * fh * fw, seed + 1, -1.0, 1.0).reshape(F, C, fh, fw)
+    td = getattr(torch, dtype)
+    op = Conv2d(B, C, F, h, w, fh, fw, mode=mode, pad=pad)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=td).contiguous())
+    out = op.forward(torch.from_numpy(img).to(device=dev, dtype=td).contiguous())
+    torch.cuda.synchronize()
+    return img, filt, out.double().cpu().numpy(), op
+
+
+@pytest.mark.gpu
+def test_conv_same_multichannel_odd_batch(dev):
+    B, C, F, h, w, fh, fw = 3, 2, 2, 64, 80, 7, 5
+    img, filt, out, op = _conv(dev, B, C, F, h, w, fh, fw, "same", 11)
+    assert out.shape == (B, F, h, w)
+    for b in range(B):
+        for f in range(F):
+            ref = sum(oracle.conv2d_same(img[b, c], filt[f, c]) for c in range(C))
+            l2, mx = oracle.err_l2max(out[b, f], ref)
+            assert l2 <= 1e-4 and mx <= 1e-3, (b, f, l2, mx)
+    print("pad", op.pad)
+
+
+@pytest.mark.gpu
+def test_conv_valid_correlation_reference_config(dev):
+    """main.cpp:320-352: 256x256 signal, 13x13 filters, 16 filters, 1 channel."""
+    B, C, F, h, w, k = 1, 1, 16, 256, 256, 13
+    img, filt, out, op = _conv(dev, B, C, F, h, w, k, k, "valid_corr", 21, dtype="float64")
+    assert out.shape == (1, F, h - k + 1, w - k + 1)
+    worst = 0.0
+    for f in range(F):
+        ref = oracle.corr2d_valid(img[0, 0], filt[f, 0])
+        l2, mx = oracle.err_l2max(out[0, f], ref)
+        worst = max(worst, l2)
+        assert l2 <= 1e-4 and mx <= 1e-3, (f, l2, mx)
+    print(f"valid corr 256^2 x 16 filters: worst rel_l2={worst:.2e}, pad={op.pad}")
+
+
+@pytest.mark.gpu
+def test_conv_same_2048_gaussian_pad2176(dev):
+    """2048x2048 fp32 U[0,1] images, 31x31 Gaussian (sigma 5, sum 1) x (1 +- 1% noise),
+    padding 2176 = 2048 + 30 rounded up."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d, good_size
+
+    assert good_size(2078) == 2176
+    B, k = 2, 31
+    yy, xx = np.mgrid[:k, :k] - (k - 1) / 2
+    g = np.exp(-(xx ** 2 + yy ** 2) / (2 * 5.0 ** 2))
+    g /= g.sum()
+    g *= 1 + oracle.uniform(k * k, 5, -0.01, 0.01).reshape(k, k)
+    img = oracle.uniform(B * 2048 * 2048, 6, 0.0, 1.0).reshape(B, 1, 2048, 2048).astype(np.float32)
+    op = Conv2d(B, 1, 1, 2048, 2048, k, k, mode="same")
+    assert op.pad == (2176, 2176)
+    op.set_filters(torch.from_numpy(g.reshape(1, 1, k, k)).to(dev, torch.float32).contiguous())
+    out = op.forward(torch.from_numpy(img).to(dev).contiguous())
+    torch.cuda.synchronize()
+    out = out.double().cpu().numpy()
+    rows = list(range(0, 16)) + list(range(1016, 1032)) + list(range(2032, 2048))
+    for b in range(B):
+        ref = np.concatenate([oracle.conv2d_same(img[b, 0].astype(np.float64), g, rows=(r0, r0 + 16))
+                              for r0 in (0, 1016, 2032)])
+        l2, mx = oracle.err_l2max(out[b, 0, rows], ref)
+        print(f"2048^2 image {b}: rel_l2={l2:.2e} rel_max={mx:.2e}")
+        assert l2 <= 1e-4 and mx <= 1e-3
