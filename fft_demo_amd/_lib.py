@@ -118,6 +118,9 @@ SIGNATURES = {
     "mfft_exec": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mfft_exec_events": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int]),
     "mfft_input_permutation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int]),
+    "mfft_graph_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp]),
+    "mfft_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "mfft_graph_destroy": (ctypes.c_int, [_vp]),
     "mfft_stage_convert": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _i64, _i64, _vp, ctypes.c_int, _vp,
                                           ctypes.c_int, ctypes.c_double, _vp]),
     "mfft_last_error": (ctypes.c_char_p, []),
@@ -148,6 +151,18 @@ SIGNATURES.update({
     "mfft_conv2d_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(ConvInfo)]),
     "mfft_conv2d_destroy": (ctypes.c_int, [_vp]),
     "mfft_good_size": (ctypes.c_int, [ctypes.c_int]),
+})
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("in_", ctypes.c_void_p), ("out", ctypes.c_void_p), ("n", ctypes.c_int), ("count", ctypes.c_int64)]
+
+
+SIGNATURES.update({
+    "mfft_batcher_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(Desc), ctypes.c_int]),
+    "mfft_batcher_exec": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, _vp]),
+    "mfft_batcher_destroy": (ctypes.c_int, [_vp]),
+    "mfft_exec_ex": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
 })
 
 

@@ -30,6 +30,9 @@ tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
 int pow2_tile_lines(int R, int TL);
 hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream);
 const void *find_small_kernel(int R, int inf, int outf);
+hipError_t launch_queue(int prec, const TileArgs *jobs, const long long *first_tile, int njobs, long long total,
+                        unsigned long long *counter, int grid, hipStream_t s);
+int queue_grid(int prec);
 hipError_t stage_convert(const void *src, int sfmt, void *dst, int dfmt, long long n_inner, long long batch,
                          const int *perm_x, int nx, const int *perm_y, int ny, double scale, hipStream_t s);
 }  // namespace mfft
@@ -473,13 +476,13 @@ static void *buf_ptr(int which, const void *in, void *out, void *work)
 }
 
 static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s, hipEvent_t *ev,
-                      int &evi)
+                      int &evi, void *work)
 {
     for (auto &ps : p->passes) {
         TileArgs a;
         memset(&a, 0, sizeof a);
-        a.in = buf_ptr(ps.src, in, out, p->work);
-        a.out = buf_ptr(ps.dst, in, out, p->work);
+        a.in = buf_ptr(ps.src, in, out, work);
+        a.out = buf_ptr(ps.dst, in, out, work);
         a.in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
         a.out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
         a.R = ps.R;
@@ -683,15 +686,18 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
     return MFFT_OK;
 }
 
-static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEvent_t *ev)
+static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEvent_t *ev, long long batch = -1,
+                    void *work = nullptr)
 {
-    std::lock_guard<std::mutex> lk(p->exec_mu);
+    std::unique_lock<std::mutex> lk(p->exec_mu, std::defer_lock);
+    if (!work) lk.lock();  // the plan's own workspace is shared by its callers
     hipStream_t s = (hipStream_t)stream;
     long long ib = p->total * fmt_bytes(p->d.in_fmt), ob = p->total * fmt_bytes(p->d.out_fmt);
+    if (batch < 0) batch = p->batch;
     int evi = 0;
-    for (long long b0 = 0; b0 < p->batch; b0 += p->chunk) {
-        long long nb = std::min(p->chunk, p->batch - b0);
-        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi);
+    for (long long b0 = 0; b0 < batch; b0 += p->chunk) {
+        long long nb = std::min(p->chunk, batch - b0);
+        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work);
         if (rc) return rc;
     }
     if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
@@ -704,6 +710,146 @@ int mfft_exec(mfft_plan p, const void *in, void *out, void *stream)
     return exec_all(p, in, out, stream, nullptr);
 }
 
+int mfft_exec_ex(mfft_plan p, const void *in, void *out, int64_t batch, void *workspace, void *stream)
+{
+    if (!p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
+    if (batch < 1 || batch > p->batch) return fail(MFFT_ERR_INVALID, "batch %lld outside 1..%lld", (long long)batch, p->batch);
+    return exec_all(p, in, out, stream, nullptr, batch, workspace);
+}
+
+// ------------------------------------------------------------------ work-queue batcher
+struct mfft_batcher_s {
+    mfft_desc d{};
+    int max_items = 0;
+    DftOps *ops = nullptr;
+    std::vector<std::pair<int, float2 *>> tables;  // stage twiddles per length
+    struct Slot {
+        TileArgs *h_jobs = nullptr, *d_jobs = nullptr;
+        long long *h_first = nullptr, *d_first = nullptr;
+        unsigned long long *counter = nullptr;
+        hipEvent_t done = nullptr;
+        bool used = false;
+    } slot[4];
+    int next = 0;
+    int grid = 0;
+    std::mutex mu;
+};
+
+int mfft_batcher_destroy(mfft_batcher b)
+{
+    if (!b) return MFFT_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->ops);
+    for (auto &t : b->tables) (void)hipFree(t.second);
+    for (auto &sl : b->slot) {
+        (void)hipHostFree(sl.h_jobs);
+        (void)hipHostFree(sl.h_first);
+        (void)hipFree(sl.d_jobs);
+        (void)hipFree(sl.d_first);
+        (void)hipFree(sl.counter);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
+    delete b;
+    return MFFT_OK;
+}
+
+int mfft_batcher_create(mfft_batcher *out, const mfft_desc *d, int max_items)
+{
+    if (!out || !d || max_items < 1) return fail(MFFT_ERR_INVALID, "bad argument");
+    int rc = check_desc(d);
+    if (rc) return rc;
+    mfft_batcher b = new mfft_batcher_s;
+    b->d = *d;
+    b->max_items = max_items;
+    hipError_t e = hipMalloc((void **)&b->ops, sizeof(DftOps));
+    if (e == hipSuccess) e = gen_dft_ops(b->ops, d->direction, 0);
+    for (auto &sl : b->slot) {
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_jobs, sizeof(TileArgs) * max_items, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_first, sizeof(long long) * (max_items + 1), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void **)&sl.d_jobs, sizeof(TileArgs) * max_items);
+        if (e == hipSuccess) e = hipMalloc((void **)&sl.d_first, sizeof(long long) * (max_items + 1));
+        if (e == hipSuccess) e = hipMalloc((void **)&sl.counter, sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        mfft_batcher_destroy(b);
+        return fail(MFFT_ERR_HIP, "batcher: %s", hipGetErrorString(e));
+    }
+    b->grid = queue_grid(d->precision);
+    *out = b;
+    return MFFT_OK;
+}
+
+static float2 *batch_table(mfft_batcher b, int n, hipStream_t s)
+{
+    for (auto &t : b->tables)
+        if (t.first == n) return t.second;
+    float2 *tab = nullptr;
+    if (hipMalloc((void **)&tab, sizeof(float2) * n) != hipSuccess) return nullptr;
+    // generated once on the null stream and waited for, so any stream may use it after
+    (void)s;
+    if (gen_twiddle(tab, n, n, 1, b->d.direction, 0) != hipSuccess) return nullptr;
+    if (hipStreamSynchronize(0) != hipSuccess) return nullptr;
+    b->tables.push_back({n, tab});
+    return tab;
+}
+
+int mfft_batcher_exec(mfft_batcher b, const mfft_batch_item *items, int nitems, void *stream)
+{
+    if (!b || (!items && nitems)) return fail(MFFT_ERR_INVALID, "null argument");
+    if (nitems < 0 || nitems > b->max_items) return fail(MFFT_ERR_INVALID, "%d items (max %d)", nitems, b->max_items);
+    if (nitems == 0) return MFFT_OK;
+    std::lock_guard<std::mutex> lk(b->mu);
+    hipStream_t s = (hipStream_t)stream;
+    auto &sl = b->slot[b->next];
+    b->next = (b->next + 1) % 4;
+    if (sl.used) HIP_TRY(hipEventSynchronize(sl.done));  // the slot's previous launch has read its jobs
+    long long total = 0;
+    for (int i = 0; i < nitems; ++i) {
+        const mfft_batch_item &it = items[i];
+        std::vector<int> rad;
+        if (it.n < 2 || it.n > kMaxTile || !line_radices(it.n, rad) || it.count < 1 || !it.in || !it.out)
+            return fail(MFFT_ERR_INVALID, "item %d: n=%d count=%lld", i, it.n, (long long)it.count);
+        float2 *tab = batch_table(b, it.n, s);
+        if (!tab) return fail(MFFT_ERR_NOMEM, "twiddle table");
+        TileArgs &a = sl.h_jobs[i];
+        memset(&a, 0, sizeof a);
+        a.in = it.in;
+        a.out = it.out;
+        a.in_fmt = b->d.in_fmt;
+        a.out_fmt = b->d.out_fmt;
+        a.R = it.n;
+        a.nstages = (int)rad.size();
+        for (int k = 0; k < a.nstages; ++k) {
+            a.radix[k] = rad[k];
+            a.stage_mul[k] = b->d.scale == MFFT_SCALE_STAGE ? 1.0f / rad[k] : 1.0f;
+        }
+        a.T = std::max(1, kMaxTile / it.n);
+        a.TL = 1;
+        a.c0 = (int)it.count;
+        a.c1 = a.c2 = 1;
+        a.in_s0 = a.out_s0 = it.n;
+        a.in_ps = a.out_ps = 1;
+        a.stage_tw = tab;
+        a.dft_ops = b->ops;
+        a.out_mul = 1.0f;
+        a.sign = b->d.direction;
+        a.in_vec = ((uintptr_t)it.in & 15) == 0;
+        a.out_vec = ((uintptr_t)it.out & 15) == 0;
+        sl.h_first[i] = total;
+        total += (it.count + a.T - 1) / a.T;
+    }
+    sl.h_first[nitems] = total;
+    HIP_TRY(hipMemcpyAsync(sl.d_jobs, sl.h_jobs, sizeof(TileArgs) * nitems, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sl.d_first, sl.h_first, sizeof(long long) * (nitems + 1), hipMemcpyHostToDevice, s));
+    int grid = (int)std::min<long long>(b->grid, total);
+    HIP_TRY(launch_queue(b->d.precision, sl.d_jobs, sl.d_first, nitems, total, sl.counter, grid, s));
+    HIP_TRY(hipEventRecord(sl.done, s));
+    sl.used = true;
+    return MFFT_OK;
+}
+
 int mfft_exec_events(mfft_plan p, const void *in, void *out, void *stream, void **events, int nevents)
 {
     if (!p || !in || !out || !events) return fail(MFFT_ERR_INVALID, "null argument");
@@ -711,6 +857,55 @@ int mfft_exec_events(mfft_plan p, const void *in, void *out, void *stream, void 
     long long need = nchunks * (long long)p->passes.size() + 1;
     if (nevents < need) return fail(MFFT_ERR_INVALID, "need %lld events, got %d", need, nevents);
     return exec_all(p, in, out, stream, (hipEvent_t *)events);
+}
+
+struct mfft_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int mfft_graph_create(mfft_graph *graph, mfft_plan p, const void *in, void *out, void *stream)
+{
+    if (!graph || !p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
+    *graph = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    hipStream_t cap = s;
+    bool own = false;
+    if (!cap) {  // the legacy default stream cannot be captured
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        own = true;
+    }
+    mfft_graph g = new mfft_graph_s;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? exec_all(p, in, out, cap, nullptr) : fail(MFFT_ERR_HIP, "begin capture: %s", hipGetErrorString(e));
+    hipError_t e2 = hipStreamEndCapture(cap, &g->graph);
+    if (!rc && e2 != hipSuccess) rc = fail(MFFT_ERR_HIP, "end capture: %s", hipGetErrorString(e2));
+    if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = fail(MFFT_ERR_HIP, "graph instantiate");
+    if (own) (void)hipStreamDestroy(cap);
+    if (rc) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    *graph = g;
+    return MFFT_OK;
+}
+
+int mfft_graph_launch(mfft_graph g, void *stream)
+{
+    if (!g) return fail(MFFT_ERR_INVALID, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return MFFT_OK;
+}
+
+int mfft_graph_destroy(mfft_graph g)
+{
+    if (!g) return MFFT_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return MFFT_OK;
 }
 
 int mfft_input_permutation(mfft_plan p, int axis, int *rev, int n)

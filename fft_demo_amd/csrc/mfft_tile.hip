@@ -593,13 +593,12 @@ __global__ void __launch_bounds__(kThreads) fft_tile_pow2(TileArgs a)
     copy_out(lds, G, a, o);
 }
 
-// Any line length made of radices {16, 2, 3, 4, 5, 8, 17} (runtime schedule).
+// Any line length made of radices {16, 2, 3, 4, 5, 8, 17} (runtime schedule): the body
+// of one tile, shared by the per-plan kernel and the work-queue kernel.
 template <int PREC>
-__global__ void __launch_bounds__(kThreads) fft_tile_generic(TileArgs a)
+__device__ __forceinline__ void generic_tile(const TileArgs &a, long long tile, float2 *lds)
 {
-    extern __shared__ float2 lds[];
     GeoR G{a.R, a.T, a.TL};
-    long long tile = xcd_remap(blockIdx.x, gridDim.x);
     TileOrigin o = tile_origin(a, tile);
     copy_in(lds, G, a, o);
     int s = 1;
@@ -623,6 +622,88 @@ __global__ void __launch_bounds__(kThreads) fft_tile_generic(TileArgs a)
         s *= r;
     }
     copy_out(lds, G, a, o);
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(kThreads) fft_tile_generic(TileArgs a)
+{
+    extern __shared__ float2 lds[];
+    generic_tile<PREC>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
+}
+
+// Many transforms of uneven length and batch count in one launch: jobs[j] describes one
+// request (its tiles are first_tile[j] .. first_tile[j+1]-1); every workgroup takes the
+// next tile from a device-side counter until the queue is empty, so each wave ends once
+// all tiles are handed out.
+template <int PREC>
+__global__ void __launch_bounds__(kThreads) fft_queue_kernel(const TileArgs *jobs, const long long *first_tile,
+                                                             int njobs, long long total, unsigned long long *counter)
+{
+    extern __shared__ float2 lds[];
+    __shared__ long long s_tile;
+    __shared__ int s_job;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            long long t = (long long)atomicAdd(counter, 1ull);
+            int lo = 0, hi = njobs - 1;
+            while (lo < hi) {  // last job with first_tile <= t
+                int mid = (lo + hi + 1) >> 1;
+                if (first_tile[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            s_tile = t;
+            s_job = lo;
+        }
+        __syncthreads();
+        const long long t = s_tile;
+        const int j = s_job;
+        __syncthreads();  // every thread has read the shared slot before it is reused
+        if (t >= total) break;
+        generic_tile<PREC>(jobs[j], t - first_tile[j], lds);
+        __syncthreads();
+    }
+}
+
+__global__ void queue_reset_kernel(unsigned long long *counter) { *counter = 0ull; }
+
+hipError_t launch_queue(int prec, const TileArgs *jobs, const long long *first_tile, int njobs, long long total,
+                        unsigned long long *counter, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, counter);
+    size_t lds = (size_t)tile_lds_bytes(kMaxTile);
+    switch (prec) {
+    case PREC_FP16:
+        hipLaunchKernelGGL(fft_queue_kernel<PREC_FP16>, dim3(grid), dim3(kThreads), lds, s, jobs, first_tile, njobs,
+                           total, counter);
+        break;
+    case PREC_BF16:
+        hipLaunchKernelGGL(fft_queue_kernel<PREC_BF16>, dim3(grid), dim3(kThreads), lds, s, jobs, first_tile, njobs,
+                           total, counter);
+        break;
+    case PREC_SPLIT:
+        hipLaunchKernelGGL(fft_queue_kernel<PREC_SPLIT>, dim3(grid), dim3(kThreads), lds, s, jobs, first_tile, njobs,
+                           total, counter);
+        break;
+    default:
+        hipLaunchKernelGGL(fft_queue_kernel<PREC_FP32>, dim3(grid), dim3(kThreads), lds, s, jobs, first_tile, njobs,
+                           total, counter);
+        break;
+    }
+    return hipGetLastError();
+}
+
+int queue_grid(int prec)
+{
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256;
+    const void *fn = prec == PREC_FP16    ? (const void *)fft_queue_kernel<PREC_FP16>
+                     : prec == PREC_BF16  ? (const void *)fft_queue_kernel<PREC_BF16>
+                     : prec == PREC_SPLIT ? (const void *)fft_queue_kernel<PREC_SPLIT>
+                                          : (const void *)fft_queue_kernel<PREC_FP32>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, tile_lds_bytes(kMaxTile)) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb * ncu;
 }
 
 // ------------------------------------------------------------------ launch table

@@ -118,6 +118,33 @@ int mfft_exec(mfft_plan plan, const void *in, void *out, void *stream);
  * can time every kernel of the exec inside its own timed region. */
 int mfft_exec_events(mfft_plan plan, const void *in, void *out, void *stream, void **events, int nevents);
 
+/* Work-queue batching: many transforms of uneven length (n <= 4096, radices
+ * {16, 8, 4, 2, 3, 5, 17}) and batch count in ONE launch of a persistent kernel that takes
+ * tiles from a device-side counter. Items are contiguous batches in the batcher's formats. */
+typedef struct mfft_batch_item {
+    const void *in;
+    void *out;
+    int n;
+    int64_t count;
+} mfft_batch_item;
+typedef struct mfft_batcher_s *mfft_batcher;
+int mfft_batcher_create(mfft_batcher *b, const mfft_desc *d, int max_items);
+int mfft_batcher_exec(mfft_batcher b, const mfft_batch_item *items, int nitems, void *stream);
+int mfft_batcher_destroy(mfft_batcher b);
+
+/* As mfft_exec, for the first `batch` transforms only (batch <= the plan's), using the
+ * caller's workspace (>= mfft_info.workspace_bytes bytes, or NULL for the plan's own):
+ * one plan can then serve several streams at once, each with its own workspace. */
+int mfft_exec_ex(mfft_plan plan, const void *in, void *out, int64_t batch, void *workspace, void *stream);
+
+/* hipGraph-captured exec: the plan's launches for fixed (in, out) pointers are captured
+ * once and replayed with one graph launch (no per-launch host work). The graph holds no
+ * memset nodes. Destroy before the plan. */
+typedef struct mfft_graph_s *mfft_graph;
+int mfft_graph_create(mfft_graph *graph, mfft_plan plan, const void *in, void *out, void *stream);
+int mfft_graph_launch(mfft_graph graph, void *stream);
+int mfft_graph_destroy(mfft_graph graph);
+
 /* Digit-reversal table of the plan's radix list (gen_rev, tcfft_half.cu:1351-1374):
  * position i of a digit-reversed buffer holds natural element rev[i]. For 2D plans,
  * axis 0 = x (rev_x), axis 1 = y (rev_y). */

@@ -137,3 +137,14 @@ def test_compat_cpp_program():
         fields = dict(kv.split("=") for kv in line.split()[1:] if "=" in kv)
         assert fields["err"] == "0"
         assert float(fields["rel_l2"]) <= 4e-3, line
+
+
+@pytest.mark.gpu
+def test_conv_cpp_program_reference_interface():
+    """fft_conv2d_half from include/tensorcore_fft_conv2d.h in a C++ program (main.cpp's
+    accuracy_conv shape: 256x256 signal, 16 filters of 13x13, 'valid' correlation)."""
+    exe = os.path.join(ROOT, "fft_demo_amd", "lib", "conv_main")
+    assert os.path.exists(exe), "build the library first"
+    r = subprocess.run([exe, "256", "13", "16"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
