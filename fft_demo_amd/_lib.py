@@ -41,6 +41,7 @@ class Desc(ctypes.Structure):
         ("workspace_limit", ctypes.c_int64),
         ("split_hint", ctypes.c_int),
         ("mid_fmt", ctypes.c_int),
+        ("external_workspace", ctypes.c_int),
     ]
 
 
@@ -148,6 +149,7 @@ SIGNATURES.update({
     "mfft_conv2d_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(ConvDesc)]),
     "mfft_conv2d_set_filters": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "mfft_conv2d_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "mfft_conv2d_forward_select": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "mfft_conv2d_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(ConvInfo)]),
     "mfft_conv2d_destroy": (ctypes.c_int, [_vp]),
     "mfft_good_size": (ctypes.c_int, [ctypes.c_int]),
@@ -163,6 +165,44 @@ SIGNATURES.update({
     "mfft_batcher_exec": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, _vp]),
     "mfft_batcher_destroy": (ctypes.c_int, [_vp]),
     "mfft_exec_ex": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+})
+
+
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("streams", ctypes.c_int), ("precision", ctypes.c_int),
+                ("max_batch_1d", ctypes.c_int), ("max_batch_2d", ctypes.c_int), ("coalesce_max", ctypes.c_int),
+                ("slab_bytes", ctypes.c_int64), ("plain", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("direction", ctypes.c_int), ("n", ctypes.c_int), ("nx", ctypes.c_int),
+                ("ny", ctypes.c_int), ("batch", ctypes.c_int64), ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("filter_id", ctypes.c_int), ("tag", ctypes.c_int64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_int64), ("status", ctypes.c_int), ("submit_us", ctypes.c_double),
+                ("start_us", ctypes.c_double), ("end_us", ctypes.c_double), ("out_scale", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("coalesced", ctypes.c_int64), ("static_bytes", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64),
+                ("peak_bytes", ctypes.c_int64)]
+
+
+REQ_FFT1D, REQ_FFT2D, REQ_CONV2D = 1, 2, 3
+SIGNATURES.update({
+    "mfft_server_desc_default": (ctypes.c_int, [ctypes.POINTER(ServerDesc)]),
+    "mfft_server_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(ServerDesc)]),
+    "mfft_server_add_filter": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                              ctypes.c_int]),
+    "mfft_server_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Request)]),
+    "mfft_server_poll": (ctypes.c_int, [_vp, ctypes.POINTER(Completion), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "mfft_server_wait_idle": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mfft_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
+    "mfft_server_destroy": (ctypes.c_int, [_vp]),
+    "mfft_server_last_error": (ctypes.c_char_p, []),
 })
 
 

@@ -197,10 +197,11 @@ int mfft_conv2d_set_filters(mfft_conv2d op, const void *filters, int real_fmt, v
     return MFFT_OK;
 }
 
-int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, void *stream)
+static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, int fsel, void *stream)
 {
     if (!op || !in || !out) return cfail(MFFT_ERR_INVALID, "null argument");
     if (!op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
+    if (fsel >= op->d.filters) return cfail(MFFT_ERR_INVALID, "filter %d of %d", fsel, op->d.filters);
     std::lock_guard<std::mutex> lk(op->mu);
     hipStream_t s = (hipStream_t)stream;
     const mfft_conv2d_desc &d = op->d;
@@ -209,15 +210,29 @@ int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, i
         return cfail(MFFT_ERR_HIP, "pack");
     int rc = mfft_exec(op->fwd_img, op->X, op->X, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    if (mfft::conv_pointwise(op->X, op->H, op->Y, op->pairs, d.channels, d.filters, (long long)op->Ph * op->Pw, s) !=
-        hipSuccess)
+    const long long P = (long long)op->Ph * op->Pw;
+    const int nf = fsel >= 0 ? 1 : d.filters;
+    const float2 *H = fsel >= 0 ? op->H + (long long)fsel * d.channels * P : op->H;
+    if (mfft::conv_pointwise(op->X, H, op->Y, op->pairs, d.channels, nf, P, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "pointwise");
-    rc = mfft_exec(op->inv, op->Y, op->Y, s);
+    rc = mfft_exec_ex(op->inv, op->Y, op->Y, op->pairs * nf, nullptr, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    if (mfft::conv_crop(op->Y, out, out_fmt == MFFT_REAL_F64, d.batch, d.filters, op->oh, op->ow, op->Ph, op->Pw,
-                        op->oy, op->ox, 1.0f, s) != hipSuccess)
+    if (mfft::conv_crop(op->Y, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph, op->Pw, op->oy,
+                        op->ox, 1.0f, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
+}
+
+int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, void *stream)
+{
+    return conv_run(op, in, in_fmt, out, out_fmt, -1, stream);
+}
+
+int mfft_conv2d_forward_select(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, int filter_index,
+                               void *stream)
+{
+    if (filter_index < 0) return cfail(MFFT_ERR_INVALID, "filter index");
+    return conv_run(op, in, in_fmt, out, out_fmt, filter_index, stream);
 }
 
 int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info)

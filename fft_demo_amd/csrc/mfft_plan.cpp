@@ -187,10 +187,25 @@ static bool factorable(long long n)
 
 // Append the passes of a length-L DFT along an axis. Input element (line, j) is at
 // line_in + j*ps_in, output element (line, k) at line_out + k*ps_out.
+static long long odd_part(long long L)
+{
+    while (L > 0 && L % 2 == 0) L /= 2;
+    return L;
+}
+
+// Odd factors the streaming kernel takes as a pass of its own.
+static bool small_odd(long long o) { return o == 3 || o == 5 || o == 9 || o == 15 || o == 17; }
+
 static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, long long ps_out,
                      std::vector<Level> lines, int src, int dst, int perm_axis, int split_hint)
 {
-    if (single_pass_ok((int)std::min<long long>(L, 1 << 30), ps_in, lines)) {
+    // odd * 2^a (2176 = 17 x 128): the odd factor in a streaming pass, the power of two on
+    // the matrix-core kernels, instead of one generic mixed-radix tile pass.
+    // split_hint = 1 keeps the single pass.
+    const long long odd = odd_part(L);
+    const bool odd_split = perm_axis < 0 && split_hint != 1 && odd > 1 && small_odd(odd) && L / odd >= 16 &&
+                           L / odd <= kMaxTile;
+    if (!odd_split && single_pass_ok((int)std::min<long long>(L, 1 << 30), ps_in, lines)) {
         PassSpec p;
         p.R = (int)L;
         p.ps_in = ps_in;
@@ -202,14 +217,22 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
         out.push_back(p);
         return MFFT_OK;
     }
-    if (dst == BUF_WORK)
-        return fail(MFFT_ERR_UNSUPPORTED, "an inner axis longer than %d points is not supported", kMaxTile);
+    // pass A's destination: the workspace, or -- when this axis must end in the workspace
+    // (rows of a 2D transform) -- the output buffer, which is free until the last axis
+    int a_dst = BUF_WORK;
+    if (dst == BUF_WORK) {
+        if (!odd_split || src != BUF_IN)
+            return fail(MFFT_ERR_UNSUPPORTED, "an inner axis longer than %d points is not supported", kMaxTile);
+        a_dst = BUF_OUT;
+    }
     if (perm_axis >= 0)
         return fail(MFFT_ERR_UNSUPPORTED,
                     "digit-reversed input is supported for axes done in one pass (length <= %d)", kMaxTile);
     // choose L1 (strided pass, <= 256) and L2 = L / L1
     long long L1 = 0;
-    if (split_hint > 1 && L % split_hint == 0 && split_hint <= 256 && factorable(split_hint)) {
+    if (odd_split) {
+        L1 = odd;
+    } else if (split_hint > 1 && L % split_hint == 0 && split_hint <= 256 && factorable(split_hint)) {
         L1 = split_hint;
     } else if (is_pow2(L) && ps_in != 1 && L % 256 == 0 && L / 256 <= 256) {
         // strided axis (2D columns): both passes are strided; give the second the
@@ -242,14 +265,14 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     a.tw_N = (int)L;
     a.tw_level = (int)a.lines.size() - 1;
     a.src = src;
-    a.dst = BUF_WORK;
+    a.dst = a_dst;
     out.push_back(a);
     // remaining: DFT_L2 over n2 for each (line, k1); output at k1 + L1*k2
-    // input side of the sub-problem reads the workspace laid out like the input
+    // input side of the sub-problem reads pass A's output, laid out like the input
     std::vector<Level> sub2;
     for (auto &l : lines) sub2.push_back(Level{l.count, l.in_stride, l.out_stride, l.batch});
     sub2.push_back(Level{L1, L2 * ps_in, ps_out, false});
-    return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, BUF_WORK, dst, -1, 0);
+    return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, a_dst, dst, -1, 1);
 }
 
 // Choose tile geometry and kernel for every pass.
@@ -287,7 +310,8 @@ static int finish_passes(mfft_plan_s *p)
         }
         ps.generic = !pow2;
         // short strided DFTs: streaming kernel, one thread per line
-        if (ps.perm_axis < 0 && ps.ps_in != 1 && (ps.R == 2 || ps.R == 4 || ps.R == 8 || ps.R == 16)) {
+        if (ps.perm_axis < 0 && ps.ps_in != 1 &&
+            (ps.R == 2 || ps.R == 4 || ps.R == 8 || ps.R == 16 || small_odd(ps.R))) {
             int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
             int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
             const void *fn = find_small_kernel(ps.R, in_fmt, out_fmt);
@@ -452,7 +476,7 @@ static int finalize_plan(mfft_plan_s *p)
         long long c = std::max(1LL, limit / std::max(1LL, per));
         p->chunk = std::min(p->batch, c);
         p->work_bytes = p->chunk * per;
-        if (alloc_dev(p, &p->work, (size_t)p->work_bytes)) return MFFT_ERR_NOMEM;
+        if (!p->d.external_workspace && alloc_dev(p, &p->work, (size_t)p->work_bytes)) return MFFT_ERR_NOMEM;
     }
     return build_tables(p);
 }
@@ -694,6 +718,7 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
     hipStream_t s = (hipStream_t)stream;
     long long ib = p->total * fmt_bytes(p->d.in_fmt), ob = p->total * fmt_bytes(p->d.out_fmt);
     if (batch < 0) batch = p->batch;
+    if (!work && !p->work && p->work_bytes > 0) return fail(MFFT_ERR_INVALID, "plan needs a workspace (mfft_exec_ex)");
     int evi = 0;
     for (long long b0 = 0; b0 < batch; b0 += p->chunk) {
         long long nb = std::min(p->chunk, batch - b0);

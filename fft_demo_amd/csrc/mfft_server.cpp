@@ -1,0 +1,558 @@
+// Continuous-batching request server for mixed 1D FFT / 2D FFT / conv2d requests
+// (C ABI in include/mfft.h). The reference has no serving layer; its nearest idea is the
+// multi-stream host-staged batch split of fft_2d_c2c (tensorcore_fft_conv2d.cpp:79-146).
+//
+// One scheduler thread:
+//   - takes newly submitted requests (submit never blocks on the GPU);
+//   - retires finished work (hipEventQuery), freeing its HBM footprint;
+//   - admits pending requests in arrival order while their footprint fits the HBM budget
+//     (requests that do not fit are passed over, smaller ones behind them go first);
+//   - coalesces the admitted small 1D transforms (n <= 4096) into ONE launch of the
+//     work-queue kernel, and sends every other request to the least-loaded stream, so
+//     requests that arrive while others are on the GPU overlap with them on the device.
+// Plans, twiddle tables and filter spectra are built once and stay resident; each stream
+// owns a workspace slab, so plans are shared by all streams (mfft_exec_ex).
+// Plain mode (the baseline): fp32 plans, one request at a time on one stream, each
+// completed before the next starts.
+#include "../../include/mfft.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+thread_local std::string s_err;
+
+int sfail(int code, const char *fmt, ...)
+{
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    s_err = buf;
+    return code;
+}
+
+double now_us()
+{
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+    mfft_request r{};
+    double t_submit = 0, t_start = 0;
+    long long need = 0;
+    int stream = -1;
+    hipEvent_t ev = nullptr;  // shared by the requests of one coalesced launch
+    bool owns_ev = false;
+    double out_scale = 1.0;
+    int status = 0;
+    bool admitted = false;
+};
+
+struct FilterHost {
+    int id, fh, fw;
+    std::vector<float> w;
+};
+
+}  // namespace
+
+struct mfft_server_s {
+    mfft_server_desc d{};
+    int prec = MFFT_PREC_SPLIT_FP16;
+    int fmt = MFFT_FMT_C16;
+    std::vector<hipStream_t> streams;
+    std::vector<void *> slabs;
+    std::vector<int> outstanding;  // requests in flight per stream
+    long long slab_bytes = 0;
+    std::atomic<long long> static_bytes{0};  // slabs + conv operators + spectra
+    mfft_batcher batcher[2] = {nullptr, nullptr};  // forward, inverse
+    std::map<std::tuple<int, int, int, int>, mfft_plan> plans;  // (kind, nx, ny, dir)
+    std::vector<FilterHost> filters;
+    struct mfft_server_convs *conv = nullptr;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Req *> incoming;
+    std::deque<Req *> pending;
+    std::vector<Req *> inflight;
+    std::vector<mfft_completion> done;
+    std::atomic<long long> used{0};  // request I/O of admitted, unfinished requests
+    long long peak = 0;              // largest static_bytes + used seen at an admission
+    long long submitted = 0, completed = 0, launches = 0, coalesced = 0;
+    double busy_since = 0;
+    std::atomic<bool> stop{false};
+    int device = 0;
+    std::thread sched;
+    std::string error;
+};
+
+namespace {
+
+long long io_bytes(const mfft_server_s *S, const mfft_request &r)
+{
+    const int eb = S->fmt == MFFT_FMT_C32 ? 8 : 4;
+    if (r.kind == MFFT_REQ_FFT1D) return 2LL * r.n * r.batch * eb;
+    if (r.kind == MFFT_REQ_FFT2D) return 2LL * r.nx * r.ny * r.batch * eb;
+    return 2LL * r.nx * r.ny * 4;  // conv: fp32 image in and out
+}
+
+mfft_plan get_plan(mfft_server_s *S, int kind, int nx, int ny, int dir)
+{
+    auto key = std::make_tuple(kind, nx, ny, dir);
+    auto it = S->plans.find(key);
+    if (it != S->plans.end()) return it->second;
+    mfft_desc d;
+    mfft_desc_default(&d);
+    d.precision = S->prec;
+    d.direction = dir;
+    d.in_fmt = d.out_fmt = S->fmt;
+    d.scale = S->prec == MFFT_PREC_FP32 ? MFFT_SCALE_NONE : MFFT_SCALE_STAGE;
+    d.workspace_limit = S->slab_bytes;
+    d.external_workspace = 1;  // the streams' slabs
+    mfft_plan p = nullptr;
+    int rc = kind == MFFT_REQ_FFT1D ? mfft_plan_create_1d(&p, nx, S->d.max_batch_1d, &d)
+                                    : mfft_plan_create_2d(&p, nx, ny, S->d.max_batch_2d, &d);
+    if (rc) return nullptr;
+    S->plans[key] = p;
+    return p;
+}
+
+// Filters whose sizes have the same parity share a frame (the class's largest height and
+// width): a filter placed at the centre of a larger zero frame gives the same 'same'-mode
+// result, so ONE conv operator per image shape and parity class holds every filter spectrum
+// of the class. Each operator is bound to one stream (its workspace is its own).
+struct ConvSlot {
+    mfft_conv2d op = nullptr;
+    std::vector<int> ids;
+    int stream = 0;
+};
+
+const FilterHost *find_filter(const mfft_server_s *S, int id);
+
+}  // namespace
+
+struct mfft_server_convs {
+    std::map<std::tuple<int, int, int, int>, ConvSlot> slots;  // (h, w, fh & 1, fw & 1)
+    int next_stream = 0;
+};
+
+namespace {
+
+ConvSlot *get_conv(mfft_server_s *S, int h, int w, int filter_id, int *index)
+{
+    const FilterHost *f = find_filter(S, filter_id);
+    if (!f) return nullptr;
+    const int ph = f->fh & 1, pw = f->fw & 1;
+    auto key = std::make_tuple(h, w, ph, pw);
+    auto it = S->conv->slots.find(key);
+    if (it == S->conv->slots.end()) {
+        int Fh = 0, Fw = 0;
+        std::vector<const FilterHost *> cls;
+        for (auto &g : S->filters)
+            if ((g.fh & 1) == ph && (g.fw & 1) == pw) {
+                cls.push_back(&g);
+                Fh = std::max(Fh, g.fh);
+                Fw = std::max(Fw, g.fw);
+            }
+        ConvSlot slot;
+        std::vector<float> stack(cls.size() * (size_t)Fh * Fw, 0.f);
+        for (size_t k = 0; k < cls.size(); ++k) {
+            const FilterHost *g = cls[k];
+            const int oy = (Fh - g->fh) / 2, ox = (Fw - g->fw) / 2;
+            for (int y = 0; y < g->fh; ++y)
+                for (int x = 0; x < g->fw; ++x)
+                    stack[(k * Fh + oy + y) * (size_t)Fw + ox + x] = g->w[(size_t)y * g->fw + x];
+            slot.ids.push_back(g->id);
+        }
+        mfft_conv2d_desc cd;
+        mfft_conv2d_desc_default(&cd);
+        cd.batch = 1;
+        cd.channels = 1;
+        cd.filters = (int)cls.size();
+        cd.h = h;
+        cd.w = w;
+        cd.fh = Fh;
+        cd.fw = Fw;
+        cd.mode = MFFT_CONV_SAME;
+        cd.precision = S->prec;
+        cd.workspace_limit = S->slab_bytes;
+        slot.stream = S->conv->next_stream++ % (int)S->streams.size();
+        if (mfft_conv2d_create(&slot.op, &cd)) return nullptr;
+        void *dfilt = nullptr;
+        if (hipMalloc(&dfilt, stack.size() * sizeof(float)) != hipSuccess) {
+            mfft_conv2d_destroy(slot.op);
+            return nullptr;
+        }
+        hipStream_t s = S->streams[slot.stream];
+        bool ok = hipMemcpy(dfilt, stack.data(), stack.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+                  mfft_conv2d_set_filters(slot.op, dfilt, MFFT_REAL_F32, s) == MFFT_OK &&
+                  hipStreamSynchronize(s) == hipSuccess;
+        (void)hipFree(dfilt);
+        if (!ok) {
+            mfft_conv2d_destroy(slot.op);
+            return nullptr;
+        }
+        mfft_conv2d_info inf;
+        mfft_conv2d_info_get(slot.op, &inf);
+        S->static_bytes += inf.device_bytes;
+        it = S->conv->slots.emplace(key, slot).first;
+    }
+    const std::vector<int> &ids = it->second.ids;
+    auto p = std::find(ids.begin(), ids.end(), filter_id);
+    if (p == ids.end()) return nullptr;
+    *index = (int)(p - ids.begin());
+    return &it->second;
+}
+
+const FilterHost *find_filter(const mfft_server_s *S, int id)
+{
+    for (auto &f : S->filters)
+        if (f.id == id) return &f;
+    return nullptr;
+}
+
+int pick_stream(mfft_server_s *S)
+{
+    int best = 0;
+    for (int i = 1; i < (int)S->streams.size(); ++i)
+        if (S->outstanding[i] < S->outstanding[best]) best = i;
+    return best;
+}
+
+// Launch one non-coalesced request on stream `st`.
+int launch_one(mfft_server_s *S, Req *q, int st)
+{
+    hipStream_t s = S->streams[st];
+    const mfft_request &r = q->r;
+    int rc = MFFT_OK;
+    if (r.kind == MFFT_REQ_FFT1D || r.kind == MFFT_REQ_FFT2D) {
+        mfft_plan p = r.kind == MFFT_REQ_FFT1D ? get_plan(S, r.kind, r.n, 0, r.direction)
+                                               : get_plan(S, r.kind, r.nx, r.ny, r.direction);
+        if (!p) return sfail(MFFT_ERR_UNSUPPORTED, "no plan");
+        mfft_info inf;
+        mfft_plan_info(p, &inf);
+        q->out_scale = inf.out_scale;
+        rc = mfft_exec_ex(p, r.in, r.out, r.batch, S->slabs[st], s);
+    } else {
+        int idx = 0;
+        ConvSlot *c = get_conv(S, r.nx, r.ny, r.filter_id, &idx);
+        if (!c) return sfail(MFFT_ERR_UNSUPPORTED, "conv operator for filter %d", r.filter_id);
+        st = c->stream;
+        s = S->streams[st];
+        rc = mfft_conv2d_forward_select(c->op, r.in, MFFT_REAL_F32, r.out, MFFT_REAL_F32, idx, s);
+    }
+    if (rc) return rc;
+    if (hipEventCreateWithFlags(&q->ev, hipEventDisableTiming) != hipSuccess) return MFFT_ERR_HIP;
+    q->owns_ev = true;
+    if (hipEventRecord(q->ev, s) != hipSuccess) return MFFT_ERR_HIP;
+    q->stream = st;
+    S->outstanding[st]++;
+    S->launches++;
+    return MFFT_OK;
+}
+
+void complete(mfft_server_s *S, Req *q, int status)
+{
+    mfft_completion c;
+    c.tag = q->r.tag;
+    c.status = status;
+    c.submit_us = q->t_submit;
+    c.start_us = q->t_start;
+    c.end_us = now_us();
+    c.out_scale = q->out_scale;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->done.push_back(c);
+        S->completed++;
+    }
+    if (q->admitted) S->used -= q->need;
+    if (q->stream >= 0) S->outstanding[q->stream]--;
+    if (q->owns_ev && q->ev) (void)hipEventDestroy(q->ev);
+    delete q;
+}
+
+void scheduler(mfft_server_s *S)
+{
+    (void)hipSetDevice(S->device);
+    const bool plain = S->d.plain != 0;
+    std::vector<Req *> small;
+    std::vector<mfft_batch_item> items;
+    while (true) {
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->incoming.empty() && S->pending.empty() && S->inflight.empty()) {
+                if (S->stop) break;
+                S->cv.wait_for(lk, std::chrono::microseconds(200));
+            }
+            while (!S->incoming.empty()) {
+                S->pending.push_back(S->incoming.front());
+                S->incoming.pop_front();
+            }
+        }
+        // retire
+        for (size_t i = 0; i < S->inflight.size();) {
+            Req *q = S->inflight[i];
+            hipError_t e = hipEventQuery(q->ev);
+            if (e == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            // requests of one coalesced launch share the event: retire them all
+            hipEvent_t ev = q->ev;
+            for (size_t j = 0; j < S->inflight.size();) {
+                if (S->inflight[j]->ev == ev) {
+                    Req *r = S->inflight[j];
+                    S->inflight.erase(S->inflight.begin() + j);
+                    bool last_owner = r->owns_ev;
+                    r->owns_ev = false;
+                    complete(S, r, e == hipSuccess ? MFFT_OK : MFFT_ERR_HIP);
+                    if (last_owner) (void)hipEventDestroy(ev);
+                } else {
+                    ++j;
+                }
+            }
+            i = 0;
+        }
+        // admit
+        small.clear();
+        items.clear();
+        const long long budget = S->d.hbm_budget;
+        for (auto it = S->pending.begin(); it != S->pending.end();) {
+            if (plain && !S->inflight.empty()) break;
+            Req *q = *it;
+            q->need = io_bytes(S, q->r);
+            if (S->static_bytes + S->used + q->need > budget) {
+                if (q->need + S->static_bytes > budget) {  // can never fit: reject
+                    it = S->pending.erase(it);
+                    complete(S, q, MFFT_ERR_NOMEM);
+                    continue;
+                }
+                ++it;
+                continue;
+            }
+            q->t_start = now_us();
+            q->admitted = true;
+            S->used += q->need;
+            S->peak = std::max(S->peak, S->static_bytes + S->used);
+            const bool is_small = !plain && q->r.kind == MFFT_REQ_FFT1D && q->r.n <= 4096 &&
+                                  (int)small.size() < S->d.coalesce_max && q->r.direction == MFFT_FORWARD;
+            if (is_small) {
+                small.push_back(q);
+                items.push_back(mfft_batch_item{q->r.in, q->r.out, q->r.n, q->r.batch});
+                q->out_scale = 1.0 / q->r.n;
+                it = S->pending.erase(it);
+                continue;
+            }
+            int st = plain ? 0 : pick_stream(S);
+            int rc = launch_one(S, q, st);
+            it = S->pending.erase(it);
+            if (rc) {
+                complete(S, q, rc);
+                continue;
+            }
+            S->inflight.push_back(q);
+            if (plain) break;
+        }
+        if (!small.empty()) {
+            int st = pick_stream(S);
+            int rc = mfft_batcher_exec(S->batcher[0], items.data(), (int)items.size(), S->streams[st]);
+            hipEvent_t ev = nullptr;
+            if (!rc && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+                hipEventRecord(ev, S->streams[st]) == hipSuccess) {
+                S->launches++;
+                S->coalesced += (long long)small.size();
+                for (size_t k = 0; k < small.size(); ++k) {
+                    small[k]->ev = ev;
+                    small[k]->owns_ev = k == 0;
+                    small[k]->stream = st;
+                    S->outstanding[st]++;
+                    S->inflight.push_back(small[k]);
+                }
+            } else {
+                for (Req *q : small) complete(S, q, rc ? rc : MFFT_ERR_HIP);
+            }
+        }
+        if (plain && !S->inflight.empty()) {
+            (void)hipStreamSynchronize(S->streams[0]);  // one request at a time
+        } else if (!S->inflight.empty() && S->pending.empty()) {
+            std::this_thread::sleep_for(std::chrono::microseconds(5));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfft_server_desc_default(mfft_server_desc *d)
+{
+    if (!d) return MFFT_ERR_INVALID;
+    memset(d, 0, sizeof *d);
+    d->hbm_budget = 64LL << 30;
+    d->streams = 4;
+    d->precision = MFFT_PREC_SPLIT_FP16;
+    d->max_batch_1d = 64;
+    d->max_batch_2d = 16;
+    d->coalesce_max = 256;
+    d->slab_bytes = 512LL << 20;
+    return MFFT_OK;
+}
+
+int mfft_server_create(mfft_server *out, const mfft_server_desc *d)
+{
+    if (!out || !d) return sfail(MFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->streams < 1 || d->streams > 16 || d->hbm_budget <= 0 || d->max_batch_1d < 1 || d->max_batch_2d < 1)
+        return sfail(MFFT_ERR_INVALID, "bad descriptor");
+    mfft_server S = new mfft_server_s;
+    S->conv = new mfft_server_convs;
+    S->d = *d;
+    (void)hipGetDevice(&S->device);
+    if (S->d.plain) {
+        S->d.streams = 1;
+        S->prec = MFFT_PREC_FP32;
+        S->fmt = MFFT_FMT_C32;
+    } else {
+        S->prec = d->precision;
+        S->fmt = d->precision == MFFT_PREC_FP32 ? MFFT_FMT_C32 : d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16 : MFFT_FMT_C16;
+    }
+    S->slab_bytes = d->slab_bytes > 0 ? d->slab_bytes : (512LL << 20);
+    S->streams.resize(S->d.streams);
+    S->slabs.resize(S->d.streams, nullptr);
+    S->outstanding.assign(S->d.streams, 0);
+    for (int i = 0; i < S->d.streams; ++i) {
+        if (hipStreamCreateWithFlags(&S->streams[i], hipStreamNonBlocking) != hipSuccess ||
+            hipMalloc(&S->slabs[i], S->slab_bytes) != hipSuccess) {
+            mfft_server_destroy(S);
+            return sfail(MFFT_ERR_NOMEM, "streams / workspace slabs");
+        }
+        S->static_bytes += S->slab_bytes;
+    }
+    if (!S->d.plain) {
+        mfft_desc bd;
+        mfft_desc_default(&bd);
+        bd.precision = S->prec;
+        bd.in_fmt = bd.out_fmt = S->fmt;
+        bd.scale = MFFT_SCALE_STAGE;
+        if (mfft_batcher_create(&S->batcher[0], &bd, S->d.coalesce_max)) {
+            std::string m = mfft_last_error();
+            mfft_server_destroy(S);
+            return sfail(MFFT_ERR_HIP, "%s", m.c_str());
+        }
+    }
+    S->sched = std::thread(scheduler, S);
+    *out = S;
+    return MFFT_OK;
+}
+
+int mfft_server_add_filter(mfft_server S, int filter_id, const float *weights, int fh, int fw)
+{
+    if (!S || !weights || fh < 1 || fw < 1) return sfail(MFFT_ERR_INVALID, "bad argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->incoming.empty() || !S->pending.empty() || S->submitted != S->completed)
+        return sfail(MFFT_ERR_INVALID, "filters are registered before requests are submitted");
+    FilterHost f{filter_id, fh, fw, std::vector<float>(weights, weights + (size_t)fh * fw)};
+    S->filters.push_back(f);
+    return MFFT_OK;
+}
+
+int mfft_server_submit(mfft_server S, const mfft_request *r)
+{
+    if (!S || !r || !r->in || !r->out) return sfail(MFFT_ERR_INVALID, "null argument");
+    if (r->kind < MFFT_REQ_FFT1D || r->kind > MFFT_REQ_CONV2D) return sfail(MFFT_ERR_INVALID, "kind");
+    if ((r->kind == MFFT_REQ_FFT1D && (r->batch < 1 || r->batch > S->d.max_batch_1d)) ||
+        (r->kind == MFFT_REQ_FFT2D && (r->batch < 1 || r->batch > S->d.max_batch_2d)))
+        return sfail(MFFT_ERR_INVALID, "batch outside the server's plans");
+    Req *q = new Req;
+    q->r = *r;
+    if (q->r.direction == 0) q->r.direction = MFFT_FORWARD;
+    q->t_submit = now_us();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->incoming.push_back(q);
+        S->submitted++;
+    }
+    S->cv.notify_one();
+    return MFFT_OK;
+}
+
+int mfft_server_poll(mfft_server S, mfft_completion *out, int max, int *n)
+{
+    if (!S || !n || (max > 0 && !out)) return sfail(MFFT_ERR_INVALID, "bad argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    int k = std::min<int>(max, (int)S->done.size());
+    for (int i = 0; i < k; ++i) out[i] = S->done[i];
+    S->done.erase(S->done.begin(), S->done.begin() + k);
+    *n = k;
+    return MFFT_OK;
+}
+
+int mfft_server_wait_idle(mfft_server S, double timeout_s)
+{
+    if (!S) return sfail(MFFT_ERR_INVALID, "null server");
+    double t0 = now_us();
+    while (true) {
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->submitted == S->completed) return MFFT_OK;
+        }
+        if (timeout_s > 0 && now_us() - t0 > timeout_s * 1e6) return sfail(MFFT_ERR_INVALID, "timeout");
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
+}
+
+int mfft_server_get_stats(mfft_server S, mfft_server_stats *st)
+{
+    if (!S || !st) return sfail(MFFT_ERR_INVALID, "bad argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    st->submitted = S->submitted;
+    st->completed = S->completed;
+    st->launches = S->launches;
+    st->coalesced = S->coalesced;
+    st->static_bytes = S->static_bytes;
+    st->inflight_bytes = S->used;
+    st->peak_bytes = S->peak;
+    return MFFT_OK;
+}
+
+int mfft_server_destroy(mfft_server S)
+{
+    if (!S) return MFFT_OK;
+    S->stop = true;
+    S->cv.notify_all();
+    if (S->sched.joinable()) S->sched.join();
+    (void)hipDeviceSynchronize();
+    if (S->conv)
+        for (auto &kv : S->conv->slots) mfft_conv2d_destroy(kv.second.op);
+    delete S->conv;
+    for (auto &kv : S->plans) mfft_plan_destroy(kv.second);
+    for (auto &b : S->batcher)
+        if (b) mfft_batcher_destroy(b);
+    for (size_t i = 0; i < S->streams.size(); ++i) {
+        if (S->slabs[i]) (void)hipFree(S->slabs[i]);
+        if (S->streams[i]) (void)hipStreamDestroy(S->streams[i]);
+    }
+    for (Req *q : S->pending) delete q;
+    for (Req *q : S->incoming) delete q;
+    delete S;
+    return MFFT_OK;
+}
+
+const char *mfft_server_last_error(void) { return s_err.c_str(); }
+
+}  // extern "C"

@@ -1,8 +1,11 @@
-// Streaming kernel for short strided DFTs (R = 2, 4, 8, 16 points per line) -- the small
-// first pass of a four-step split along a strided axis (e.g. 1024 = 4 x 256 columns of a
-// 2D transform). One thread owns one line: it loads the R points (consecutive threads take
-// consecutive lines, so every load is coalesced), transforms them in fp32 registers,
-// applies the four-step twiddle and the stage scale, and stores -- no LDS, one HBM pass.
+// Streaming kernel for short strided DFTs -- the small first pass of a four-step split
+// along a strided axis: R = 2, 4, 8, 16 (e.g. 1024 = 4 x 256 columns of a 2D transform)
+// and the odd R = 3, 5, 9, 15, 17 that take the odd factor of lengths odd * 2^a (e.g.
+// 2176 = 17 x 128), so the power-of-two rest runs on the matrix-core kernels. One thread
+// owns one line: it loads the R points (consecutive threads take consecutive lines, so
+// every load is coalesced), transforms them in fp32 registers, applies the four-step
+// twiddle and the stage scale, and stores -- no LDS, one HBM pass. A thread reads and
+// writes the same R positions, so the pass may run in place.
 #include "mfft_common.h"
 
 namespace mfft {
@@ -91,6 +94,44 @@ __device__ __forceinline__ void dft(float2 *v, int sg)
     }
 }
 
+// Odd R: y_k = x_0 + sum_j S_j c_jk + i sum_j D_j t_jk and y_{R-k} = x_0 + sum_j S_j c_jk -
+// i sum_j D_j t_jk (j, k = 1..(R-1)/2), with S_j = x_j + x_{R-j}, D_j = x_j - x_{R-j} and
+// W_R^m = c_m + i t_m from the pass table (FP64-generated, carries the transform's sign):
+// (R-1)^2 real multiply-adds per line instead of 4 (R-1)^2.
+template <int R>
+__device__ __forceinline__ void dft_odd(float2 *v, const float2 *__restrict__ tab)
+{
+    constexpr int H = (R - 1) / 2;
+    float2 w[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) w[m] = tab[m];
+    float2 S[H + 1], D[H + 1];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        S[j] = make_float2(v[j].x + v[R - j].x, v[j].y + v[R - j].y);
+        D[j] = make_float2(v[j].x - v[R - j].x, v[j].y - v[R - j].y);
+    }
+    float2 y0 = v[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) y0 = make_float2(y0.x + S[j].x, y0.y + S[j].y);
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        float2 a = v[0], b = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const float2 t = w[(j * k) % R];
+            a.x += S[j].x * t.x;
+            a.y += S[j].y * t.x;
+            b.x += D[j].x * t.y;
+            b.y += D[j].y * t.y;
+        }
+        // i * b = (-b.y, b.x)
+        v[k] = make_float2(a.x - b.y, a.y + b.x);
+        v[R - k] = make_float2(a.x + b.y, a.y - b.x);
+    }
+    v[0] = y0;
+}
+
 // Lines are (l0, l1, l2) as in TileArgs; one thread per line.
 template <int R, int INF, int OUTF>
 __global__ void __launch_bounds__(256) small_cols(TileArgs a)
@@ -107,7 +148,10 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
     float2 v[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = ld<INF>(a.in, ib + (long long)j * a.in_ps);
-    dft<R>(v, a.sign);
+    if constexpr (R & 1)
+        dft_odd<R>(v, a.stage_tw);
+    else
+        dft<R>(v, a.sign);
     const unsigned mult = (unsigned)(a.tw_level ? l1 : l0);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -128,6 +172,11 @@ const void *pick(int R)
     case 4: return (const void *)small_cols<4, INF, OUTF>;
     case 8: return (const void *)small_cols<8, INF, OUTF>;
     case 16: return (const void *)small_cols<16, INF, OUTF>;
+    case 3: return (const void *)small_cols<3, INF, OUTF>;
+    case 5: return (const void *)small_cols<5, INF, OUTF>;
+    case 9: return (const void *)small_cols<9, INF, OUTF>;
+    case 15: return (const void *)small_cols<15, INF, OUTF>;
+    case 17: return (const void *)small_cols<17, INF, OUTF>;
     default: return nullptr;
     }
 }

@@ -67,6 +67,8 @@ typedef struct mfft_desc {
     int mid_fmt;              /* format of intermediates between passes: -1 = the precision's
                                  default (fp16 -> C16, bf16 -> CBF16, split/fp32 -> C32), or an
                                  MFFT_FMT_*; split fp16 with C16 rounds once per pass boundary */
+    int external_workspace;   /* 1: the plan allocates no workspace; every exec passes one
+                                 (mfft_exec_ex) of at least mfft_info.workspace_bytes */
 } mfft_desc;
 
 typedef struct mfft_plan_s *mfft_plan;
@@ -199,10 +201,76 @@ int mfft_conv2d_create(mfft_conv2d *op, const mfft_conv2d_desc *d);
 int mfft_conv2d_set_filters(mfft_conv2d op, const void *filters, int real_fmt, void *stream);
 /* in: device [B][C][h][w], out: device [B][F][out_h][out_w] (MFFT_REAL_* each) */
 int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, void *stream);
+/* as forward, with only resident filter `filter_index` (out: [B][1][out_h][out_w]) */
+int mfft_conv2d_forward_select(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, int filter_index,
+                               void *stream);
 int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info);
 int mfft_conv2d_destroy(mfft_conv2d op);
 /* smallest transform length >= n that the planner supports well (2^a * {1,3,5,17}) */
 int mfft_good_size(int n);
+
+/* ------------------------------------------------------------------------------------
+ * Continuous-batching request server (no counterpart in the reference). Mixed 1D FFT,
+ * 2D FFT and conv2d requests on device buffers are submitted without blocking; a
+ * scheduler admits them while the in-flight device footprint fits `hbm_budget`, runs
+ * admitted work concurrently on `streams` HIP streams, coalesces small 1D transforms
+ * (n <= 4096) into one work-queue launch, and keeps plans, twiddles and filter spectra
+ * resident. plain = 1 gives the baseline: fp32, one request at a time, each completed
+ * before the next starts.
+ * FFT requests: in/out are interleaved complex in the server's format (fp16 for split /
+ * fp16 precision, fp32 for plain); out = out_scale * DFT(in) (mfft_completion.out_scale).
+ * conv2d requests: in/out are fp32 real [nx][ny] images ('same' convolution with the
+ * registered filter `filter_id`). Filters are registered before the first request; those
+ * whose sizes have the same parity share one operator per image shape (each filter centred
+ * in the class's largest frame), so all of their spectra stay resident together.
+ * ------------------------------------------------------------------------------------ */
+enum { MFFT_REQ_FFT1D = 1, MFFT_REQ_FFT2D = 2, MFFT_REQ_CONV2D = 3 };
+
+typedef struct mfft_server_desc {
+    int64_t hbm_budget;    /* bytes: workspace slabs + conv operators + in-flight request I/O */
+    int streams;           /* concurrent HIP streams (1..16) */
+    int precision;         /* MFFT_PREC_* (ignored in plain mode: fp32) */
+    int max_batch_1d, max_batch_2d;
+    int coalesce_max;      /* small 1D requests per work-queue launch */
+    int64_t slab_bytes;    /* workspace per stream (default 512 MiB) */
+    int plain;
+} mfft_server_desc;
+
+typedef struct mfft_request {
+    int kind;              /* MFFT_REQ_* */
+    int direction;         /* MFFT_FORWARD (default) / MFFT_INVERSE; FFT requests */
+    int n;                 /* 1D length */
+    int nx, ny;            /* 2D shape / conv image (nx rows of ny) */
+    int64_t batch;         /* FFT transforms in the request */
+    const void *in;
+    void *out;
+    int filter_id;         /* conv2d */
+    int64_t tag;
+} mfft_request;
+
+typedef struct mfft_completion {
+    int64_t tag;
+    int status;            /* MFFT_OK or an error (MFFT_ERR_NOMEM: larger than the budget) */
+    double submit_us, start_us, end_us;   /* host steady clock */
+    double out_scale;
+} mfft_completion;
+
+typedef struct mfft_server_stats {
+    int64_t submitted, completed, launches, coalesced;
+    int64_t static_bytes, inflight_bytes;
+    int64_t peak_bytes;    /* largest static + in-flight footprint at any admission */
+} mfft_server_stats;
+
+typedef struct mfft_server_s *mfft_server;
+int mfft_server_desc_default(mfft_server_desc *d);
+int mfft_server_create(mfft_server *server, const mfft_server_desc *d);
+int mfft_server_add_filter(mfft_server server, int filter_id, const float *weights, int fh, int fw);
+int mfft_server_submit(mfft_server server, const mfft_request *r);
+int mfft_server_poll(mfft_server server, mfft_completion *out, int max, int *n);
+int mfft_server_wait_idle(mfft_server server, double timeout_s);
+int mfft_server_get_stats(mfft_server server, mfft_server_stats *stats);
+int mfft_server_destroy(mfft_server server);
+const char *mfft_server_last_error(void);
 
 /* Message of the calling thread's last error ("" if none). */
 const char *mfft_last_error(void);

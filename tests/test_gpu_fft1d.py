@@ -46,6 +46,17 @@ def test_sizes_split(dev, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp32"])
+@pytest.mark.parametrize("n", [1 << 17, 1 << 18, 1 << 19, 1 << 20])
+def test_long_lengths(dev, n, prec):
+    """Lengths past one 65536-point four-step (the serving mix draws N up to 2^20)."""
+    y, ref, plan = _run(dev, n, 3, prec, seed=n % 1000)
+    l2, mx = rel_errors(y, ref)
+    print(f"N={n} b=3 {prec}: rel_l2={l2:.3e} rel_max={mx:.3e} passes={len(plan.info['passes'])}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n", [256, 4096, 65536])
 def test_inverse_split(dev, n):
     y, ref, plan = _run(dev, n, 4, "split_fp16", direction=1)
@@ -69,3 +80,26 @@ def test_fp32_io_split(dev):
     l2, mx = rel_errors(y, ref)
     print(f"split c32 io: {l2:.3e} {mx:.3e}")
     assert l2 <= 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp32"])
+@pytest.mark.parametrize("n", [48, 80, 96, 240, 576, 1088, 2176, 2560, 12288, 69632])
+def test_mixed_radix(dev, n, prec):
+    """odd * 2^a lengths: the odd factor (3, 5, 9, 15, 17) as a streaming pass, the power
+    of two on the matrix-core kernels."""
+    b = max(1, min(16, (1 << 18) // n))
+    y, ref, plan = _run(dev, n, b, prec, seed=n % 89)
+    l2, mx = rel_errors(y, ref)
+    print(f"N={n} b={b} {prec}: rel_l2={l2:.3e} rel_max={mx:.3e} passes={[p['R'] for p in plan.info['passes']]}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [240, 2176])
+def test_mixed_radix_single_pass(dev, n):
+    """split_hint = 1 keeps the single mixed-radix tile pass."""
+    y, ref, plan = _run(dev, n, 8, "split_fp16", seed=3, split_hint=1)
+    assert len(plan.info["passes"]) == 1
+    l2, mx = rel_errors(y, ref)
+    assert l2 <= 1e-3 and mx <= 4e-3

@@ -19,7 +19,7 @@ def _plan(nx, ny, b, **kw):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nx,ny,b", [(256, 256, 4), (64, 128, 3), (512, 512, 2), (1024, 1024, 2), (2048, 2048, 1),
-                                     (16, 4096, 2), (4096, 32, 1)])
+                                     (16, 4096, 2), (4096, 32, 1), (2176, 2176, 1), (576, 640, 2), (96, 48, 3)])
 def test_2d_split(dev, nx, ny, b):
     plan = _plan(nx, ny, b, precision="split_fp16")
     x = complex_input((b, nx, ny), nx + ny)
@@ -59,6 +59,11 @@ def test_2d_inverse_and_fp32(dev):
     for prec, tol, fmt in [("split_fp16", 1e-3, "c16"), ("fp32", 1e-5, "c32")]:
         plan = _plan(256, 512, 2, precision=prec, direction=1)
         x = complex_input((2, 256, 512), 9, fmt)
+        y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
+        l2, _ = rel_errors(y, oracle.fft2d(x, sign=1))
+        assert l2 <= tol, (prec, l2)
+        plan = _plan(1088, 576, 1, precision=prec, direction=1)  # 17 * 64 by 9 * 64
+        x = complex_input((1, 1088, 576), 10, fmt)
         y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
         l2, _ = rel_errors(y, oracle.fft2d(x, sign=1))
         assert l2 <= tol, (prec, l2)
