@@ -150,6 +150,8 @@ SIGNATURES.update({
     "mfft_conv2d_set_filters": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "mfft_conv2d_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
     "mfft_conv2d_forward_select": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "mfft_conv2d_forward_two": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, _vp]),
     "mfft_conv2d_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(ConvInfo)]),
     "mfft_conv2d_destroy": (ctypes.c_int, [_vp]),
     "mfft_good_size": (ctypes.c_int, [ctypes.c_int]),
@@ -171,7 +173,7 @@ SIGNATURES.update({
 class ServerDesc(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_int64), ("streams", ctypes.c_int), ("precision", ctypes.c_int),
                 ("max_batch_1d", ctypes.c_int), ("max_batch_2d", ctypes.c_int), ("coalesce_max", ctypes.c_int),
-                ("slab_bytes", ctypes.c_int64), ("plain", ctypes.c_int)]
+                ("slab_bytes", ctypes.c_int64), ("plain", ctypes.c_int), ("mid_fmt", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -188,7 +190,7 @@ class Completion(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("coalesced", ctypes.c_int64), ("static_bytes", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64),
-                ("peak_bytes", ctypes.c_int64)]
+                ("peak_bytes", ctypes.c_int64), ("paired", ctypes.c_int64)]
 
 
 REQ_FFT1D, REQ_FFT2D, REQ_CONV2D = 1, 2, 3

@@ -68,6 +68,25 @@ class Conv2d:
                                               ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_forward")
         return out
 
+    def forward_two(self, x0, x1, f0, f1, out0=None, out1=None, stream=None):
+        """Single-channel operators: x0 (*) filter f0 and x1 (*) filter f1 in one complex
+        transform (x0, x1: [h][w] images of one dtype)."""
+        import torch
+
+        d = self.desc
+        for x in (x0, x1):
+            assert x.is_cuda and x.is_contiguous() and x.numel() == d.h * d.w and x.dtype == x0.dtype
+        oh, ow = self.out_shape[2:]
+        out0 = torch.empty((oh, ow), dtype=x0.dtype, device=x0.device) if out0 is None else out0
+        out1 = torch.empty((oh, ow), dtype=x0.dtype, device=x0.device) if out1 is None else out1
+        s = stream if stream is not None else torch.cuda.current_stream(x0.device)
+        f = self._fmt(x0)
+        L.check(L.load().mfft_conv2d_forward_two(self._h, ctypes.c_void_p(x0.data_ptr()), ctypes.c_void_p(x1.data_ptr()),
+                                                  f, ctypes.c_void_p(out0.data_ptr()), ctypes.c_void_p(out1.data_ptr()),
+                                                  f, int(f0), int(f1), ctypes.c_void_p(s.cuda_stream)),
+                "mfft_conv2d_forward_two")
+        return out0, out1
+
     def close(self):
         if getattr(self, "_h", None):
             L.load().mfft_conv2d_destroy(self._h)

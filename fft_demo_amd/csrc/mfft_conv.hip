@@ -93,6 +93,52 @@ __global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh
     out[i] = (T)((b & 1 ? v.y : v.x) * scale);
 }
 
+// Two real images a, b (separate buffers, [h][w]) -> one complex image a + i b, zero-padded.
+template <typename T>
+__global__ void pack_two_kernel(const T *a, const T *b, float2 *dst, int h, int w, int Ph, int Pw, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % Pw), y = (int)(i / Pw);
+    float2 v = make_float2(0.f, 0.f);
+    if (y < h && x < w) {
+        long long off = (long long)y * w + x;
+        v = make_float2((float)a[off], (float)b[off]);
+    }
+    dst[i] = v;
+}
+
+// Spectrum X of a + i b (a, b real): A = (X(k) + conj X(-k)) / 2 and B = (X(k) - conj X(-k))
+// / 2i are the spectra of a and b, so Y = A H0 + i B H1 = ((X + X*)/2) H0 + ((X - X*)/2) H1
+// (X* = conj X(-k)) transforms back to (a conv h0) + i (b conv h1): two convolutions with
+// different filters from one forward and one inverse transform.
+__global__ void pointwise_two_kernel(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)Ph * Pw) return;
+    int v = (int)(i % Pw), u = (int)(i / Pw);
+    int uq = u ? Ph - u : 0, vq = v ? Pw - v : 0;
+    float2 xp = X[i], xq = X[(long long)uq * Pw + vq];
+    xq.y = -xq.y;
+    float2 s = make_float2(0.5f * (xp.x + xq.x), 0.5f * (xp.y + xq.y));
+    float2 d = make_float2(0.5f * (xp.x - xq.x), 0.5f * (xp.y - xq.y));
+    float2 h0 = H0[i], h1 = H1[i];
+    Y[i] = make_float2(s.x * h0.x - s.y * h0.y + d.x * h1.x - d.y * h1.y,
+                       s.x * h0.y + s.y * h0.x + d.x * h1.y + d.y * h1.x);
+}
+
+// out0 = Re Y, out1 = Im Y over the output window at (oy, ox).
+template <typename T>
+__global__ void crop_two_kernel(const float2 *Y, T *out0, T *out1, int oh, int ow, int Pw, int oy, int ox, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % ow), y = (int)(i / ow);
+    float2 v = Y[(long long)(y + oy) * Pw + (x + ox)];
+    out0[i] = (T)v.x;
+    out1[i] = (T)v.y;
+}
+
 static inline unsigned nblocks(long long n) { return (unsigned)((n + 255) / 256); }
 
 hipError_t conv_pack_pairs(const void *src, int f64, float2 *dst, int B, int C, int h, int w, int Ph, int Pw,
@@ -139,6 +185,39 @@ hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, 
     else
         hipLaunchKernelGGL(crop_unpack_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out, B, F, oh,
                            ow, Ph, Pw, oy, ox, scale, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_pack_two(const void *a, const void *b, int f64, float2 *dst, int h, int w, int Ph, int Pw,
+                         hipStream_t s)
+{
+    long long total = (long long)Ph * Pw;
+    if (f64)
+        hipLaunchKernelGGL(pack_two_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, (const double *)a,
+                           (const double *)b, dst, h, w, Ph, Pw, total);
+    else
+        hipLaunchKernelGGL(pack_two_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, (const float *)a,
+                           (const float *)b, dst, h, w, Ph, Pw, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_pointwise_two(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(pointwise_two_kernel, dim3(nblocks((long long)Ph * Pw)), dim3(256), 0, s, X, H0, H1, Y, Ph, Pw);
+    return hipGetLastError();
+}
+
+hipError_t conv_crop_two(const float2 *Y, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy, int ox,
+                         hipStream_t s)
+{
+    long long total = (long long)oh * ow;
+    if (f64)
+        hipLaunchKernelGGL(crop_two_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out0,
+                           (double *)out1, oh, ow, Pw, oy, ox, total);
+    else
+        hipLaunchKernelGGL(crop_two_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out0,
+                           (float *)out1, oh, ow, Pw, oy, ox, total);
     return hipGetLastError();
 }
 

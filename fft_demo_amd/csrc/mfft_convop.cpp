@@ -28,6 +28,12 @@ hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long
                           hipStream_t s);
 hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
                      float scale, hipStream_t s);
+hipError_t conv_pack_two(const void *a, const void *b, int f64, float2 *dst, int h, int w, int Ph, int Pw,
+                         hipStream_t s);
+hipError_t conv_pointwise_two(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw,
+                              hipStream_t s);
+hipError_t conv_crop_two(const float2 *Y, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy, int ox,
+                         hipStream_t s);
 }  // namespace mfft
 
 struct mfft_conv2d_s {
@@ -233,6 +239,33 @@ int mfft_conv2d_forward_select(mfft_conv2d op, const void *in, int in_fmt, void 
 {
     if (filter_index < 0) return cfail(MFFT_ERR_INVALID, "filter index");
     return conv_run(op, in, in_fmt, out, out_fmt, filter_index, stream);
+}
+
+int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, int in_fmt, void *out0, void *out1,
+                            int out_fmt, int filter0, int filter1, void *stream)
+{
+    if (!op || !in0 || !in1 || !out0 || !out1) return cfail(MFFT_ERR_INVALID, "null argument");
+    if (!op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
+    const mfft_conv2d_desc &d = op->d;
+    if (d.channels != 1) return cfail(MFFT_ERR_UNSUPPORTED, "forward_two needs a single-channel operator");
+    if (filter0 < 0 || filter1 < 0 || filter0 >= d.filters || filter1 >= d.filters)
+        return cfail(MFFT_ERR_INVALID, "filter index");
+    std::lock_guard<std::mutex> lk(op->mu);
+    hipStream_t s = (hipStream_t)stream;
+    const long long P = (long long)op->Ph * op->Pw;
+    if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, d.h, d.w, op->Ph, op->Pw, s) != hipSuccess)
+        return cfail(MFFT_ERR_HIP, "pack");
+    int rc = mfft_exec_ex(op->fwd_img, op->X, op->X, 1, nullptr, s);
+    if (rc) return cfail(rc, "%s", mfft_last_error());
+    if (mfft::conv_pointwise_two(op->X, op->H + filter0 * P, op->H + filter1 * P, op->Y, op->Ph, op->Pw, s) !=
+        hipSuccess)
+        return cfail(MFFT_ERR_HIP, "pointwise");
+    rc = mfft_exec_ex(op->inv, op->Y, op->Y, 1, nullptr, s);
+    if (rc) return cfail(rc, "%s", mfft_last_error());
+    if (mfft::conv_crop_two(op->Y, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy, op->ox, s) !=
+        hipSuccess)
+        return cfail(MFFT_ERR_HIP, "crop");
+    return MFFT_OK;
 }
 
 int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info)

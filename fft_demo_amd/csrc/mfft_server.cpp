@@ -93,7 +93,7 @@ struct mfft_server_s {
     std::vector<mfft_completion> done;
     std::atomic<long long> used{0};  // request I/O of admitted, unfinished requests
     long long peak = 0;              // largest static_bytes + used seen at an admission
-    long long submitted = 0, completed = 0, launches = 0, coalesced = 0;
+    long long submitted = 0, completed = 0, launches = 0, coalesced = 0, paired = 0;
     double busy_since = 0;
     std::atomic<bool> stop{false};
     int device = 0;
@@ -124,6 +124,7 @@ mfft_plan get_plan(mfft_server_s *S, int kind, int nx, int ny, int dir)
     d.scale = S->prec == MFFT_PREC_FP32 ? MFFT_SCALE_NONE : MFFT_SCALE_STAGE;
     d.workspace_limit = S->slab_bytes;
     d.external_workspace = 1;  // the streams' slabs
+    if (!S->d.plain) d.mid_fmt = S->d.mid_fmt;
     mfft_plan p = nullptr;
     int rc = kind == MFFT_REQ_FFT1D ? mfft_plan_create_1d(&p, nx, S->d.max_batch_1d, &d)
                                     : mfft_plan_create_2d(&p, nx, ny, S->d.max_batch_2d, &d);
@@ -266,6 +267,28 @@ int launch_one(mfft_server_s *S, Req *q, int st)
     return MFFT_OK;
 }
 
+// Two conv requests on one operator: one complex transform carries both images.
+int launch_conv_two(mfft_server_s *S, Req *a, Req *b, ConvSlot *c, int ia, int ib)
+{
+    hipStream_t s = S->streams[c->stream];
+    int rc = mfft_conv2d_forward_two(c->op, a->r.in, b->r.in, MFFT_REAL_F32, a->r.out, b->r.out, MFFT_REAL_F32, ia, ib, s);
+    if (rc) return rc;
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return MFFT_ERR_HIP;
+    if (hipEventRecord(ev, s) != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        return MFFT_ERR_HIP;
+    }
+    a->ev = b->ev = ev;
+    a->owns_ev = true;
+    b->owns_ev = false;
+    a->stream = b->stream = c->stream;
+    S->outstanding[c->stream] += 2;
+    S->launches++;
+    S->paired += 2;
+    return MFFT_OK;
+}
+
 void complete(mfft_server_s *S, Req *q, int status)
 {
     mfft_completion c;
@@ -292,6 +315,7 @@ void scheduler(mfft_server_s *S)
     const bool plain = S->d.plain != 0;
     std::vector<Req *> small;
     std::vector<mfft_batch_item> items;
+    std::map<ConvSlot *, std::vector<std::pair<Req *, int>>> convq;  // admitted conv requests per operator
     while (true) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
@@ -331,6 +355,7 @@ void scheduler(mfft_server_s *S)
         // admit
         small.clear();
         items.clear();
+        convq.clear();
         const long long budget = S->d.hbm_budget;
         for (auto it = S->pending.begin(); it != S->pending.end();) {
             if (plain && !S->inflight.empty()) break;
@@ -358,6 +383,18 @@ void scheduler(mfft_server_s *S)
                 it = S->pending.erase(it);
                 continue;
             }
+            if (!plain && q->r.kind == MFFT_REQ_CONV2D) {
+                // held back to pair with another request on the same operator
+                int idx = 0;
+                ConvSlot *c = get_conv(S, q->r.nx, q->r.ny, q->r.filter_id, &idx);
+                it = S->pending.erase(it);
+                if (!c) {
+                    complete(S, q, sfail(MFFT_ERR_UNSUPPORTED, "conv operator for filter %d", q->r.filter_id));
+                    continue;
+                }
+                convq[c].push_back({q, idx});
+                continue;
+            }
             int st = plain ? 0 : pick_stream(S);
             int rc = launch_one(S, q, st);
             it = S->pending.erase(it);
@@ -367,6 +404,28 @@ void scheduler(mfft_server_s *S)
             }
             S->inflight.push_back(q);
             if (plain) break;
+        }
+        for (auto &kv : convq) {
+            auto &v = kv.second;
+            size_t k = 0;
+            for (; k + 1 < v.size(); k += 2) {
+                int rc = launch_conv_two(S, v[k].first, v[k + 1].first, kv.first, v[k].second, v[k + 1].second);
+                if (rc) {
+                    complete(S, v[k].first, rc);
+                    complete(S, v[k + 1].first, rc);
+                    continue;
+                }
+                S->inflight.push_back(v[k].first);
+                S->inflight.push_back(v[k + 1].first);
+            }
+            if (k < v.size()) {
+                Req *q = v[k].first;
+                int rc = launch_one(S, q, kv.first->stream);
+                if (rc)
+                    complete(S, q, rc);
+                else
+                    S->inflight.push_back(q);
+            }
         }
         if (!small.empty()) {
             int st = pick_stream(S);
@@ -410,6 +469,7 @@ int mfft_server_desc_default(mfft_server_desc *d)
     d->max_batch_2d = 16;
     d->coalesce_max = 256;
     d->slab_bytes = 512LL << 20;
+    d->mid_fmt = -1;
     return MFFT_OK;
 }
 
@@ -524,6 +584,7 @@ int mfft_server_get_stats(mfft_server S, mfft_server_stats *st)
     st->completed = S->completed;
     st->launches = S->launches;
     st->coalesced = S->coalesced;
+    st->paired = S->paired;
     st->static_bytes = S->static_bytes;
     st->inflight_bytes = S->used;
     st->peak_bytes = S->peak;

@@ -41,6 +41,8 @@ class Server:
 
     precision: "split" (split fp16, default), "fp16", "bf16" or "fp32"; plain=True gives
     the baseline (fp32, one request at a time, each finished before the next starts).
+    mid_fmt="c16" keeps the split plans' intermediates in fp16 (one more rounding per pass
+    boundary, half the intermediate traffic).
     FFT requests carry interleaved complex data in the server's format (fp16 pairs for
     split/fp16, bf16 pairs for bf16, fp32 pairs for fp32 and plain) and produce
     out_scale * DFT(in); conv2d requests carry fp32 real images.
@@ -49,7 +51,7 @@ class Server:
     PREC = {"fp16": 0, "bf16": 1, "split": 2, "fp32": 3}
 
     def __init__(self, precision="split", plain=False, hbm_budget=64 << 30, streams=4, max_batch_1d=64,
-                 max_batch_2d=16, coalesce_max=256, slab_bytes=512 << 20):
+                 max_batch_2d=16, coalesce_max=256, slab_bytes=512 << 20, mid_fmt=None):
         self.L = _lib
         self.lib = _lib.load()
         d = _lib.ServerDesc()
@@ -62,6 +64,7 @@ class Server:
         d.coalesce_max = int(coalesce_max)
         d.slab_bytes = int(slab_bytes)
         d.plain = 1 if plain else 0
+        d.mid_fmt = -1 if mid_fmt is None else {"c16": 0, "cbf16": 1, "c32": 2}[mid_fmt]
         self.desc = d
         self.plain = plain
         self.precision = "fp32" if plain else precision
@@ -173,9 +176,11 @@ class Arena:
     """Shared read-only inputs and a ring of outputs on the device for trace replay.
 
     Every request of a kind reads the same input region (uniform[-1,1] complex for FFTs,
-    uniform[0,1] images for conv) and writes its own slice of an output ring; the ring is
-    larger than the server's HBM budget admits at once, so an admitted request's output is
-    never reused before it completes.
+    uniform[0,1] images for conv) and writes its own slice of an output ring, assigned at
+    submission. While less than the ring's size of output is outstanding no two requests
+    in flight share memory; a saturation replay (everything submitted at once) wraps the
+    ring, so late requests overwrite early outputs -- timing is unaffected, and results
+    are checked by the tests with buffers of their own.
     """
 
     def __init__(self, server, trace, ring_bytes, device="cuda"):

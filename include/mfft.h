@@ -204,6 +204,15 @@ int mfft_conv2d_forward(mfft_conv2d op, const void *in, int in_fmt, void *out, i
 /* as forward, with only resident filter `filter_index` (out: [B][1][out_h][out_w]) */
 int mfft_conv2d_forward_select(mfft_conv2d op, const void *in, int in_fmt, void *out, int out_fmt, int filter_index,
                                void *stream);
+/* Two single-image convolutions in one complex transform: image in0 with filter filter0
+ * and image in1 with filter filter1 (operators with channels = 1, any batch). The images
+ * become the real and imaginary parts of one complex image; the pointwise step separates
+ * their spectra (A = (X(k) + conj X(-k))/2, B = (X(k) - conj X(-k))/2i) and forms
+ * A H0 + i B H1, so the inverse holds both results. Half the transform work of two
+ * mfft_conv2d_forward_select calls. No counterpart in the reference (its batches pair
+ * nothing). */
+int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, int in_fmt, void *out0, void *out1,
+                            int out_fmt, int filter0, int filter1, void *stream);
 int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info);
 int mfft_conv2d_destroy(mfft_conv2d op);
 /* smallest transform length >= n that the planner supports well (2^a * {1,3,5,17}) */
@@ -215,7 +224,8 @@ int mfft_good_size(int n);
  * scheduler admits them while the in-flight device footprint fits `hbm_budget`, runs
  * admitted work concurrently on `streams` HIP streams, coalesces small 1D transforms
  * (n <= 4096) into one work-queue launch, and keeps plans, twiddles and filter spectra
- * resident. plain = 1 gives the baseline: fp32, one request at a time, each completed
+ * resident; conv requests admitted together on one operator run in pairs, two images
+ * in one complex transform (mfft_conv2d_forward_two). plain = 1 gives the baseline: fp32, one request at a time, each completed
  * before the next starts.
  * FFT requests: in/out are interleaved complex in the server's format (fp16 for split /
  * fp16 precision, fp32 for plain); out = out_scale * DFT(in) (mfft_completion.out_scale).
@@ -234,6 +244,7 @@ typedef struct mfft_server_desc {
     int coalesce_max;      /* small 1D requests per work-queue launch */
     int64_t slab_bytes;    /* workspace per stream (default 512 MiB) */
     int plain;
+    int mid_fmt;           /* intermediates of the FFT plans (mfft_desc.mid_fmt; -1 = default) */
 } mfft_server_desc;
 
 typedef struct mfft_request {
@@ -259,6 +270,7 @@ typedef struct mfft_server_stats {
     int64_t submitted, completed, launches, coalesced;
     int64_t static_bytes, inflight_bytes;
     int64_t peak_bytes;    /* largest static + in-flight footprint at any admission */
+    int64_t paired;        /* conv requests that shared a transform with another */
 } mfft_server_stats;
 
 typedef struct mfft_server_s *mfft_server;

@@ -87,3 +87,30 @@ def test_conv_same_2048_gaussian_pad2176(dev):
         l2, mx = oracle.err_l2max(out[b, 0, rows], ref)
         print(f"2048^2 image {b}: rel_l2={l2:.2e} rel_max={mx:.2e}")
         assert l2 <= 1e-4 and mx <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["same", "valid_corr"])
+def test_conv_forward_two_separates_paired_images(dev, mode):
+    """Two images with different filters in one complex transform (spectra separated by
+    X(k) +- conj X(-k)); each result against the direct FP64 oracle, tolerance as above."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    h, w, k, F = 200, 150, 7, 3
+    filt = oracle.uniform(F * k * k, 41, -1.0, 1.0).reshape(F, 1, k, k)
+    imgs = [oracle.uniform(h * w, 42 + i, 0.0, 1.0).reshape(h, w) for i in range(2)]
+    op = Conv2d(1, 1, F, h, w, k, k, mode=mode)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=torch.float32).contiguous())
+    xs = [torch.from_numpy(im).to(device=dev, dtype=torch.float32).contiguous() for im in imgs]
+    for f0, f1 in [(2, 0), (1, 1)]:
+        o0, o1 = op.forward_two(xs[0], xs[1], f0, f1)
+        torch.cuda.synchronize()
+        for im, f, o in [(imgs[0], f0, o0), (imgs[1], f1, o1)]:
+            ref = oracle.conv2d_same(im, filt[f, 0]) if mode == "same" else oracle.corr2d_valid(im, filt[f, 0])
+            got = o.double().cpu().numpy()
+            l2 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            mx = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"forward_two {mode} f={f} rel_l2={l2:.2e} max={mx:.2e}")
+            assert l2 <= 1e-4 and mx <= 1e-3

@@ -1,10 +1,12 @@
 """GPU tests of the continuous-batching request server (mfft_server_* in include/mfft.h).
 
   * every result of a mixed batch of 1D FFT / 2D FFT / conv2d requests, all submitted at
-    once (so they overlap on the device and the small 1D ones share one work-queue
-    launch), equals the same request's result when it is run alone -- bit for bit (the
-    same kernels on the same data; tolerance 0) -- and each is within tolerance of the
-    FP64 oracle (split fp16: rel-L2 <= 1e-3 for FFTs, <= 1e-4 for conv);
+    once (so they overlap on the device, the small 1D ones share one work-queue launch
+    and conv requests on one operator share a complex transform in pairs), equals the
+    same request's result when it is run alone: FFT results bit for bit (the same kernels
+    on the same data; tolerance 0), conv results within rel-L2 2e-5 (a paired image
+    shares its transform's rounding with its partner) -- and each is within tolerance of
+    the FP64 oracle (split fp16: rel-L2 <= 1e-3 for FFTs, <= 1e-4 for conv);
   * admission by HBM budget: a request that can never fit is rejected with
     MFFT_ERR_NOMEM, and the footprint never exceeds the budget while the rest run;
   * plain mode (fp32, one request at a time) against the oracle (rel-L2 <= 1e-5).
@@ -92,13 +94,20 @@ def test_server_mix_equals_alone(dev):
         st = S.stats()
         print("stats", st)
         assert st["coalesced"] >= 2  # small 1D requests shared a work-queue launch
+        assert st["paired"] >= 2  # two conv requests on one operator shared a transform
         for i in range(len(reqs)):
             outs[i].zero_()
             print("alone:", i, reqs[i], flush=True)
             _submit(S, reqs, ins, outs, [i])
             _collect(S, 1)
             torch.cuda.synchronize()
-            assert torch.equal(outs[i], mixed[i]), f"request {i} {reqs[i]} differs alone vs in the mix"
+            if reqs[i][0] == 3:
+                a, m = outs[i].double(), mixed[i].double()
+                d = float((a - m).norm() / m.norm())
+                print(f"conv alone vs mix rel_l2={d:.2e}")
+                assert d <= 2e-5, f"request {i} {reqs[i]}"
+            else:
+                assert torch.equal(outs[i], mixed[i]), f"request {i} {reqs[i]} differs alone vs in the mix"
         for i, (kind, n, nx, ny, b, fid) in enumerate(reqs):
             if kind == 3:
                 got = mixed[i].double().cpu().numpy()
