@@ -11,7 +11,16 @@ and the fp16 / bf16 variants on the same sizes, torch.fft (rocFFT, fp32) as a la
 external data point, the roofline of the dominant kernel from HIP events on the exec
 stream, and the FP64 CPU baseline on the host cores.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3]
+Serving (the "serve" object of the line): the continuous-batching request server on the
+mixed request workload -- Poisson arrivals at 2000 requests/s (seed 42), 60% 1D FFT,
+25% 2D FFT, 15% conv2d against 32 resident filters, 64 GB HBM budget, split fp16 --
+replayed in real time: p50/p99 latency and throughput, the same trace through the
+server's plain mode (fp32, one request at a time), saturation throughput of both and of
+torch.fft (rocFFT) called once per request, and the highest arrival rate each sustains
+within one p99 bound. --config c5 runs the full 30 s trace; the default line carries a
+shorter one (--serve-seconds).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c5]
 """
 from __future__ import annotations
 
@@ -203,12 +212,157 @@ def _cpu_model():
     return "unknown"
 
 
+# ------------------------------------------------------------------------- serving
+def _rocfft_plain(trace, filters, dev):
+    """torch.fft (rocFFT, complex64) once per request, each finished before the next:
+    plain per-request FP32 GPU FFT launches. conv2d = irfft2(rfft2(image) * resident
+    filter spectrum) over the same padded size as the server's operator."""
+    import torch
+
+    from fft_demo_amd import serve
+    from fft_demo_amd.conv import good_size
+
+    g = torch.Generator(device="cpu").manual_seed(7)
+    maxp = max([serve.points(r) for r in trace if r[1] != serve.KIND_CONV] + [1])
+    xin = torch.view_as_complex((torch.rand(maxp, 2, generator=g) * 2 - 1).to(dev))
+    side = max(serve.SIDES_CONV)
+    img = torch.rand(side * side, generator=g).to(dev)
+    fmax = max(f.shape[0] for f in filters)
+    spec = {}
+
+    def filt_spec(s, fid):
+        key = (s, fid)
+        if key not in spec:
+            p = good_size(s + fmax - 1)
+            f = filters[fid]
+            fr = torch.zeros(p, p, device=dev)
+            o = (fmax - f.shape[0]) // 2  # centred in the common frame, as in the server
+            fr[o:o + f.shape[0], o:o + f.shape[1]] = torch.from_numpy(f).to(dev)
+            spec[key] = (p, torch.fft.rfft2(fr))
+        return spec[key]
+
+    def one(r):
+        _, kind, n, nx, ny, b, fid = r
+        if kind == serve.KIND_1D:
+            return torch.fft.fft(xin[: n * b].view(b, n))
+        if kind == serve.KIND_2D:
+            return torch.fft.fft2(xin[: nx * ny * b].view(b, nx, ny))
+        p, H = filt_spec(nx, fid)
+        c = (fmax - 1) // 2
+        y = torch.fft.irfft2(torch.fft.rfft2(img[: nx * ny].view(nx, ny), s=(p, p)) * H, s=(p, p))
+        return y[c:c + nx, c:c + ny]
+
+    for r in trace:  # every plan (rocFFT's plan cache) and filter spectrum outside the timed region
+        one(r)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for r in trace:
+        t = time.perf_counter()
+        one(r)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    span = time.perf_counter() - t0
+    import numpy as np
+
+    return {"requests": len(trace), "req_per_s": len(trace) / span,
+            "gflops": sum(serve.flops(r) for r in trace) / span / 1e9,
+            "service_p50_ms": float(np.percentile(lat, 50)), "service_p99_ms": float(np.percentile(lat, 99)),
+            "note": "torch.fft (rocFFT, complex64), one request at a time; labelled external baseline"}
+
+
+def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, log):
+    """Highest arrival rate (x1.5 steps from start_rate) whose p99 stays within the bound;
+    returns the replay result at that rate. shapes_trace covers the request shapes (plans
+    and operators are built from it before the probes)."""
+    from fft_demo_amd import serve
+
+    best = None
+    rate = start_rate
+    S = make()
+    serve.register_filters(S, filters)
+    ar = serve.Arena(S, shapes_trace, 16 << 30)
+    serve.warm(S, shapes_trace, ar)
+    try:
+        for _ in range(10):
+            tr = serve.make_trace(rate, seconds, 43)
+            r = serve.replay(S, tr, ar)
+            log.append({"rate": round(rate, 1), "p99_ms": round(r["p99_ms"], 3), "req_per_s": round(r["req_per_s"], 1)})
+            if r["p99_ms"] > target_p99_ms or r["req_per_s"] < 0.9 * rate:
+                break
+            best = r
+            rate *= 1.5
+    finally:
+        S.close()
+        del ar
+    return best
+
+
+def serve_section(seconds, sat_requests=2000, p99_bound_ms=2.0):
+    import torch
+
+    from fft_demo_amd import serve
+
+    filters = serve.make_filters()
+    trace = serve.make_trace(2000.0, seconds, 42)
+    out = {"workload": f"Poisson 2000 req/s for {seconds:g} s (seed 42), 60% 1D / 25% 2D / 15% conv2d, "
+                       f"32 resident filters, 64 GB HBM budget", "requests": len(trace)}
+    modes = {"split": dict(precision="split", mid_fmt="c16"), "plain_fp32": dict(plain=True)}
+    for name, kw in modes.items():
+        S = serve.Server(**kw)
+        serve.register_filters(S, filters)
+        ar = serve.Arena(S, trace, 16 << 30)
+        serve.warm(S, trace, ar)
+        torch.cuda.synchronize()
+        r = serve.replay(S, trace, ar)
+        sat = serve.replay(S, trace[:sat_requests], ar, time_scale=1e9)
+        st = S.stats()
+        S.close()
+        del ar
+        torch.cuda.empty_cache()
+        out[name] = {"p50_ms": round(r["p50_ms"], 4), "p99_ms": round(r["p99_ms"], 4),
+                     "req_per_s": round(r["req_per_s"], 1), "gflops": round(r["gflops"], 1),
+                     "saturation_req_per_s": round(sat["req_per_s"], 1),
+                     "saturation_gflops": round(sat["gflops"], 1),
+                     "launches": st["launches"], "coalesced": st["coalesced"], "paired": st["paired"],
+                     "static_bytes": st["static_bytes"], "peak_bytes": st["peak_bytes"]}
+    try:
+        out["plain_rocfft_external"] = _rocfft_plain(trace[:sat_requests // 2], filters, torch.device("cuda:0"))
+    except Exception as e:  # noqa: BLE001
+        out["plain_rocfft_external"] = {"error": str(e)}
+    torch.cuda.empty_cache()
+    # equal p99: the highest arrival rate each mode sustains with p99 within one bound
+    bound = p99_bound_ms
+    logs = {}
+    sus = {}
+    for name, kw in modes.items():
+        logs[name] = []
+        b = _sustained(lambda kw=kw: serve.Server(**kw), filters, trace, bound, min(seconds, 2.0), 2000.0, logs[name])
+        sus[name] = round(b["req_per_s"], 1) if b else None
+        torch.cuda.empty_cache()
+    out["equal_p99"] = {"p99_bound_ms": bound, "sustained_req_per_s": sus, "probes": logs,
+                        "ratio": round(sus["split"] / sus["plain_fp32"], 3) if sus["split"] and sus["plain_fp32"]
+                        else None}
+    out["saturation_ratio_vs_plain_fp32"] = round(out["split"]["saturation_req_per_s"] /
+                                                  out["plain_fp32"]["saturation_req_per_s"], 3)
+    rf = out["plain_rocfft_external"]
+    if "req_per_s" in rf:
+        out["saturation_ratio_vs_plain_rocfft"] = round(out["split"]["saturation_req_per_s"] / rf["req_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="trace length of the serving run (c5: 30)")
+    ap.add_argument("--serve-p99-ms", type=float, default=2.0, help="p99 bound of the equal-p99 comparison")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="headline precision only")
     args = ap.parse_args()
@@ -222,6 +376,23 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                          "launches": 5}))
+        return
+    if args.config == "c5":
+        sv = serve_section(30.0, p99_bound_ms=args.serve_p99_ms)
+        print(json.dumps({"metric": METRIC, "value": sv["split"]["p99_ms"], "unit": "ms (p99)", "n_gpus": 1,
+                          "steps": sv["requests"], "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "fp16", "data": "synthetic",
+                          "config": {"workload": sv["workload"]}, "serve": sv}))
+        return
     shape, batch, desc = CONFIGS[args.config]
     xh = make_input(shape, batch)
     sample = 4 if len(shape) == 1 else 1
@@ -286,6 +457,9 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(shape, batch, xh)
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        line["serve"] = serve_section(args.serve_seconds, p99_bound_ms=args.serve_p99_ms)
     print(json.dumps(line))
 
 

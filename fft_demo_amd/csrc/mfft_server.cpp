@@ -79,6 +79,7 @@ struct mfft_server_s {
     std::vector<hipStream_t> streams;
     std::vector<void *> slabs;
     std::vector<int> outstanding;  // requests in flight per stream
+    std::vector<hipEvent_t> free_events;  // completion events for reuse
     long long slab_bytes = 0;
     std::atomic<long long> static_bytes{0};  // slabs + conv operators + spectra
     mfft_batcher batcher[2] = {nullptr, nullptr};  // forward, inverse
@@ -131,6 +132,17 @@ mfft_plan get_plan(mfft_server_s *S, int kind, int nx, int ny, int dir)
     if (rc) return nullptr;
     S->plans[key] = p;
     return p;
+}
+
+// Streams form two lanes: requests below kBulkPoints complex points go to the fast lane,
+// larger ones to the bulk lane, so a small request never queues behind a large one on
+// its stream. Within a lane, the stream with the fewest requests in flight.
+constexpr long long kBulkPoints = 1LL << 20;
+
+long long req_points(const mfft_request &r)
+{
+    if (r.kind == MFFT_REQ_FFT1D) return (long long)r.n * r.batch;
+    return (long long)r.nx * r.ny * (r.kind == MFFT_REQ_FFT2D ? r.batch : 1);
 }
 
 // Filters whose sizes have the same parity share a frame (the class's largest height and
@@ -192,7 +204,16 @@ ConvSlot *get_conv(mfft_server_s *S, int h, int w, int filter_id, int *index)
         cd.mode = MFFT_CONV_SAME;
         cd.precision = S->prec;
         cd.workspace_limit = S->slab_bytes;
-        slot.stream = S->conv->next_stream++ % (int)S->streams.size();
+        {
+            mfft_request probe{};
+            probe.kind = MFFT_REQ_CONV2D;
+            probe.nx = h;
+            probe.ny = w;
+            const bool bulk = req_points(probe) >= kBulkPoints;
+            const int n = (int)S->streams.size(), nf = n > 1 ? n / 2 : n;
+            const int lo = bulk && n > 1 ? nf : 0, width = bulk && n > 1 ? n - nf : nf;
+            slot.stream = lo + S->conv->next_stream++ % width;
+        }
         if (mfft_conv2d_create(&slot.op, &cd)) return nullptr;
         void *dfilt = nullptr;
         if (hipMalloc(&dfilt, stack.size() * sizeof(float)) != hipSuccess) {
@@ -227,12 +248,31 @@ const FilterHost *find_filter(const mfft_server_s *S, int id)
     return nullptr;
 }
 
-int pick_stream(mfft_server_s *S)
+int pick_stream(mfft_server_s *S, bool bulk)
 {
-    int best = 0;
-    for (int i = 1; i < (int)S->streams.size(); ++i)
+    const int n = (int)S->streams.size();
+    const int nf = n > 1 ? n / 2 : n;  // fast lane [0, nf), bulk lane [nf, n)
+    int lo = 0, hi = nf;
+    if (bulk && n > 1) {
+        lo = nf;
+        hi = n;
+    }
+    int best = lo;
+    for (int i = lo + 1; i < hi; ++i)
         if (S->outstanding[i] < S->outstanding[best]) best = i;
     return best;
+}
+
+hipEvent_t take_event(mfft_server_s *S)
+{
+    if (!S->free_events.empty()) {
+        hipEvent_t e = S->free_events.back();
+        S->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
 }
 
 // Launch one non-coalesced request on stream `st`.
@@ -258,7 +298,8 @@ int launch_one(mfft_server_s *S, Req *q, int st)
         rc = mfft_conv2d_forward_select(c->op, r.in, MFFT_REAL_F32, r.out, MFFT_REAL_F32, idx, s);
     }
     if (rc) return rc;
-    if (hipEventCreateWithFlags(&q->ev, hipEventDisableTiming) != hipSuccess) return MFFT_ERR_HIP;
+    q->ev = take_event(S);
+    if (!q->ev) return MFFT_ERR_HIP;
     q->owns_ev = true;
     if (hipEventRecord(q->ev, s) != hipSuccess) return MFFT_ERR_HIP;
     q->stream = st;
@@ -273,10 +314,10 @@ int launch_conv_two(mfft_server_s *S, Req *a, Req *b, ConvSlot *c, int ia, int i
     hipStream_t s = S->streams[c->stream];
     int rc = mfft_conv2d_forward_two(c->op, a->r.in, b->r.in, MFFT_REAL_F32, a->r.out, b->r.out, MFFT_REAL_F32, ia, ib, s);
     if (rc) return rc;
-    hipEvent_t ev = nullptr;
-    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return MFFT_ERR_HIP;
+    hipEvent_t ev = take_event(S);
+    if (!ev) return MFFT_ERR_HIP;
     if (hipEventRecord(ev, s) != hipSuccess) {
-        (void)hipEventDestroy(ev);
+        S->free_events.push_back(ev);
         return MFFT_ERR_HIP;
     }
     a->ev = b->ev = ev;
@@ -305,7 +346,7 @@ void complete(mfft_server_s *S, Req *q, int status)
     }
     if (q->admitted) S->used -= q->need;
     if (q->stream >= 0) S->outstanding[q->stream]--;
-    if (q->owns_ev && q->ev) (void)hipEventDestroy(q->ev);
+    if (q->owns_ev && q->ev) S->free_events.push_back(q->ev);
     delete q;
 }
 
@@ -328,29 +369,23 @@ void scheduler(mfft_server_s *S)
                 S->incoming.pop_front();
             }
         }
-        // retire
-        for (size_t i = 0; i < S->inflight.size();) {
-            Req *q = S->inflight[i];
-            hipError_t e = hipEventQuery(q->ev);
-            if (e == hipErrorNotReady) {
-                ++i;
-                continue;
+        // retire: query each event once; requests of one coalesced or paired launch share
+        // their event and retire together (its owner returns it to the pool)
+        {
+            std::vector<Req *> keep;
+            std::vector<std::pair<Req *, hipError_t>> fin;
+            std::map<hipEvent_t, hipError_t> seen;
+            keep.reserve(S->inflight.size());
+            for (Req *q : S->inflight) {
+                auto f = seen.find(q->ev);
+                hipError_t e = f != seen.end() ? f->second : (seen[q->ev] = hipEventQuery(q->ev));
+                if (e == hipErrorNotReady)
+                    keep.push_back(q);
+                else
+                    fin.push_back({q, e});
             }
-            // requests of one coalesced launch share the event: retire them all
-            hipEvent_t ev = q->ev;
-            for (size_t j = 0; j < S->inflight.size();) {
-                if (S->inflight[j]->ev == ev) {
-                    Req *r = S->inflight[j];
-                    S->inflight.erase(S->inflight.begin() + j);
-                    bool last_owner = r->owns_ev;
-                    r->owns_ev = false;
-                    complete(S, r, e == hipSuccess ? MFFT_OK : MFFT_ERR_HIP);
-                    if (last_owner) (void)hipEventDestroy(ev);
-                } else {
-                    ++j;
-                }
-            }
-            i = 0;
+            S->inflight.swap(keep);
+            for (auto &d : fin) complete(S, d.first, d.second == hipSuccess ? MFFT_OK : MFFT_ERR_HIP);
         }
         // admit
         small.clear();
@@ -395,7 +430,7 @@ void scheduler(mfft_server_s *S)
                 convq[c].push_back({q, idx});
                 continue;
             }
-            int st = plain ? 0 : pick_stream(S);
+            int st = plain ? 0 : pick_stream(S, req_points(q->r) >= kBulkPoints);
             int rc = launch_one(S, q, st);
             it = S->pending.erase(it);
             if (rc) {
@@ -428,11 +463,10 @@ void scheduler(mfft_server_s *S)
             }
         }
         if (!small.empty()) {
-            int st = pick_stream(S);
+            int st = pick_stream(S, false);
             int rc = mfft_batcher_exec(S->batcher[0], items.data(), (int)items.size(), S->streams[st]);
-            hipEvent_t ev = nullptr;
-            if (!rc && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
-                hipEventRecord(ev, S->streams[st]) == hipSuccess) {
+            hipEvent_t ev = rc ? nullptr : take_event(S);
+            if (ev && hipEventRecord(ev, S->streams[st]) == hipSuccess) {
                 S->launches++;
                 S->coalesced += (long long)small.size();
                 for (size_t k = 0; k < small.size(); ++k) {
@@ -443,6 +477,7 @@ void scheduler(mfft_server_s *S)
                     S->inflight.push_back(small[k]);
                 }
             } else {
+                if (ev) S->free_events.push_back(ev);
                 for (Req *q : small) complete(S, q, rc ? rc : MFFT_ERR_HIP);
             }
         }
@@ -608,6 +643,7 @@ int mfft_server_destroy(mfft_server S)
         if (S->slabs[i]) (void)hipFree(S->slabs[i]);
         if (S->streams[i]) (void)hipStreamDestroy(S->streams[i]);
     }
+    for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
     for (Req *q : S->pending) delete q;
     for (Req *q : S->incoming) delete q;
     delete S;
