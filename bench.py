@@ -61,7 +61,10 @@ def make_input(shape, batch, seed=42):
 
 
 def time_plan(plan, x, out, steps, warmup, stream):
-    """Run `steps` execs with per-launch HIP events; return (total_ms, per-pass avg ms)."""
+    """Time `steps` back-to-back execs between two HIP events on the exec stream (the
+    figure reported); then a second run of `steps` execs with an event between every
+    launch gives each pass's average launch time (for the roofline). Returns
+    (total_ms of the first run, [per-pass avg ms])."""
     import torch
 
     from fft_demo_amd.hiptime import EventPool
@@ -71,16 +74,38 @@ def time_plan(plan, x, out, steps, warmup, stream):
     for _ in range(warmup):
         plan.exec(x, out, stream=stream)
     torch.cuda.synchronize()
-    pool = EventPool(steps * (nl + 1))
     lib = __import__("fft_demo_amd._lib", fromlist=["load"]).load()
+    ends = EventPool(2)
+    ends.record(0, stream.cuda_stream)
+    for _ in range(steps):
+        plan.exec(x, out, stream=stream)
+    ends.record(1, stream.cuda_stream)
     torch.cuda.synchronize()
+    total = ends.elapsed_ms(0, 1)
+    if total / steps < 0.05:
+        # launch-bound (a few microseconds per exec): capture the steps in one hipGraph and
+        # time its replay, so host launch overhead is not what is measured
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cs = torch.cuda.current_stream()  # the capture stream
+            for _ in range(steps):
+                plan.exec(x, out, stream=cs)
+        g.replay()
+        torch.cuda.synchronize()
+        ends.record(0, stream.cuda_stream)
+        g.replay()
+        ends.record(1, stream.cuda_stream)
+        torch.cuda.synchronize()
+        total = ends.elapsed_ms(0, 1)
+        del g
+    ends.close()
+    pool = EventPool(steps * (nl + 1))
     for i in range(steps):
         rc = lib.mfft_exec_events(plan._h, x.data_ptr(), out.data_ptr(), stream.cuda_stream, pool.slice(i * (nl + 1)),
                                   nl + 1)
         if rc != 0:
             raise RuntimeError(lib.mfft_last_error().decode())
     torch.cuda.synchronize()
-    total = pool.elapsed_ms(0, steps * (nl + 1) - 1)
     per_pass = [0.0] * npass
     per_pass_n = [0] * npass
     for i in range(steps):
@@ -90,7 +115,6 @@ def time_plan(plan, x, out, steps, warmup, stream):
             per_pass[k % npass] += d
             per_pass_n[k % npass] += 1
     pool.close()
-    # wall time of the steps themselves (first launch of step 0 .. end of the last step)
     return total, [per_pass[i] / max(1, per_pass_n[i]) for i in range(npass)]
 
 
@@ -111,8 +135,12 @@ def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample):
 
     import fft_demo_amd as F
 
+    from fft_demo_amd import tune
+
     prec = name
-    plan = F.Plan(shape, batch, precision=prec)
+    # every variant gets the same treatment: autotuned schedule (split point, batch chunk)
+    sched = tune.autotune(shape, batch, precision=prec)
+    plan = F.Plan(shape, batch, precision=prec, split_hint=tune.SPLIT_TUNED)
     dt = plan.in_dtype()
     x = torch.from_numpy(xh).to(device=dev, dtype=dt).contiguous()
     out = plan.empty_out(device=dev)
@@ -128,6 +156,7 @@ def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample):
         "pass_avg_ms": per_pass,
         "batch_chunk": plan.info["batch_chunk"],
         "launches_per_exec": plan.info["launches_per_exec"],
+        "schedule": sched,
     }
     del x, out
     plan.close()
@@ -210,6 +239,25 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def profiled_traffic():
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy
+    kernel of known size), from the summaries committed under profiles/: {key: bytes}, key =
+    "<kernel family>:R<points per line>:c<in bits>>c<out bits>"."""
+    out = {}
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return out
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(d, name)) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    for k, v in json.loads(line.split(":", 1)[1]).items():
+                        out[k] = (int(v), name)
+    return out
 
 
 # ------------------------------------------------------------------------- serving
@@ -415,15 +463,18 @@ def main():
     pinfo = head["passes"][k]
     algo_bytes = pinfo["points"] * (pinfo["in_elem_bytes"] + pinfo["out_elem_bytes"])
     achieved = algo_bytes / (pp[k] * 1e-3) / 1e9
+    tkey = f"{pinfo['kernel']}:R{pinfo['R']}:c{pinfo['in_elem_bytes'] * 4}>c{pinfo['out_elem_bytes'] * 4}"
+    tmeas = profiled_traffic().get(tkey)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "kernel": f"fft_tile_pow2 pass {k} (R={pinfo['R']}, lines/tile={pinfo['lines_per_tile']}, "
-                  f"lane_lines={pinfo['lane_lines']})",
+        "traffic": tmeas[0] if tmeas else None,
+        "traffic_source": f"profiles/{tmeas[1]} ({tkey})" if tmeas else None,
+        "kernel": f"pass {k}: {pinfo['kernel']} (R={pinfo['R']}, lines/tile={pinfo['lines_per_tile']}, "
+                  f"c{pinfo['in_elem_bytes'] * 4} in / c{pinfo['out_elem_bytes'] * 4} out)",
         "algorithmic_bytes_per_launch": algo_bytes,
         "avg_launch_ms": round(pp[k], 5),
     }
@@ -451,6 +502,7 @@ def main():
         "passes": head["passes"],
         "pass_avg_ms": [round(v, 5) for v in pp],
         "batch_chunk": head["batch_chunk"],
+        "schedule": head["schedule"],
         "comparators": comps,
         "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3) if "fp32" in comps else None,
         "roofline": roofline,

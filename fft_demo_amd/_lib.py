@@ -45,6 +45,10 @@ class Desc(ctypes.Structure):
     ]
 
 
+KERNEL_NAMES = {0: "tile_pow2", 1: "tile_generic", 2: "fast_rows", 3: "fast_rows_tr", 4: "fast_cols",
+                5: "stream_small"}
+
+
 class PassInfo(ctypes.Structure):
     _fields_ = [
         ("R", ctypes.c_int),
@@ -57,6 +61,7 @@ class PassInfo(ctypes.Structure):
         ("in_elem_bytes", ctypes.c_int),
         ("out_elem_bytes", ctypes.c_int),
         ("points", ctypes.c_int64),
+        ("kernel", ctypes.c_int),
     ]
 
 
@@ -96,6 +101,7 @@ class Info(ctypes.Structure):
                     "generic": bool(p.generic),
                     "in_elem_bytes": p.in_elem_bytes,
                     "out_elem_bytes": p.out_elem_bytes,
+                    "kernel": KERNEL_NAMES.get(p.kernel, str(p.kernel)),
                     "points": p.points,
                 }
                 for p in self.passes[: self.npasses]
@@ -167,6 +173,24 @@ SIGNATURES.update({
     "mfft_batcher_exec": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, _vp]),
     "mfft_batcher_destroy": (ctypes.c_int, [_vp]),
     "mfft_exec_ex": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+})
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("split_hint", ctypes.c_int), ("workspace_limit", ctypes.c_int64), ("ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double), ("candidates", ctypes.c_int)]
+
+
+SPLIT_TUNED = -1
+SIGNATURES.update({
+    "mfft_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(Desc), ctypes.c_int,
+                                     ctypes.POINTER(TuneEntry)]),
+    "mfft_tune_lookup": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                        ctypes.POINTER(Desc), ctypes.POINTER(TuneEntry)]),
+    "mfft_tune_cache_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "mfft_tune_cache_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "mfft_tune_cache_clear": (ctypes.c_int, []),
+    "mfft_tune_last_error": (ctypes.c_char_p, []),
 })
 
 

@@ -490,6 +490,7 @@ static int check_desc(const mfft_desc *d)
         return fail(MFFT_ERR_INVALID, "format");
     if (d->in_order != 0 && d->in_order != 1) return fail(MFFT_ERR_INVALID, "in_order");
     if (d->mid_fmt < -1 || d->mid_fmt > 2) return fail(MFFT_ERR_INVALID, "mid_fmt");
+    if (d->split_hint < 0) return fail(MFFT_ERR_INVALID, "split_hint %d", d->split_hint);
     return MFFT_OK;
 }
 
@@ -588,11 +589,29 @@ int mfft_desc_default(mfft_desc *d)
     return MFFT_OK;
 }
 
-int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d)
+// split_hint = MFFT_SPLIT_TUNED: take split and chunk from the autotuner's cache.
+static mfft_desc resolve_tuned(const mfft_desc *d, int nx, int ny, int64_t batch)
 {
-    if (!plan || !d) return fail(MFFT_ERR_INVALID, "null argument");
+    mfft_desc r = *d;
+    if (d->split_hint != MFFT_SPLIT_TUNED) return r;
+    r.split_hint = 0;
+    mfft_tune_entry e;
+    std::string keep = g_err;
+    if (mfft_tune_lookup(nullptr, nx, ny, batch, d, &e) == MFFT_OK) {
+        r.split_hint = e.split_hint;
+        r.workspace_limit = e.workspace_limit;
+    }
+    g_err = keep;
+    return r;
+}
+
+int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d_in)
+{
+    if (!plan || !d_in) return fail(MFFT_ERR_INVALID, "null argument");
     *plan = nullptr;
     if (n < 2 || batch < 1) return fail(MFFT_ERR_INVALID, "n=%d batch=%lld", n, (long long)batch);
+    const mfft_desc dt = resolve_tuned(d_in, n, 0, batch);
+    const mfft_desc *d = &dt;
     int rc = check_desc(d);
     if (rc) return rc;
     mfft_plan_s *p = new mfft_plan_s;
@@ -620,11 +639,13 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     return MFFT_OK;
 }
 
-int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d)
+int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in)
 {
-    if (!plan || !d) return fail(MFFT_ERR_INVALID, "null argument");
+    if (!plan || !d_in) return fail(MFFT_ERR_INVALID, "null argument");
     *plan = nullptr;
     if (nx < 2 || ny < 2 || batch < 1) return fail(MFFT_ERR_INVALID, "nx=%d ny=%d batch=%lld", nx, ny, (long long)batch);
+    const mfft_desc dt = resolve_tuned(d_in, nx, ny, batch);
+    const mfft_desc *d = &dt;
     int rc = check_desc(d);
     if (rc) return rc;
     mfft_plan_s *p = new mfft_plan_s;
@@ -646,11 +667,12 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
     long long NN = (long long)nx * ny;
     // rows (y axis, contiguous): lines = (x, b)
     std::vector<Level> rows{Level{nx, ny, ny, false}, Level{batch, NN, NN, true}};
-    rc = plan_axis(p->passes, ny, 1, 1, rows, BUF_IN, BUF_WORK, rev ? 1 : -1, 0);
+    const int odd_hint = d->split_hint == 1 ? 1 : 0;  // 1 keeps odd * 2^a axes in one pass
+    rc = plan_axis(p->passes, ny, 1, 1, rows, BUF_IN, BUF_WORK, rev ? 1 : -1, odd_hint);
     // columns (x axis, stride ny): lines = (y, b)
     if (!rc) {
         std::vector<Level> cols{Level{ny, 1, 1, false}, Level{batch, NN, NN, true}};
-        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, 0);
+        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, odd_hint);
     }
     if (!rc) rc = finalize_plan(p);
     if (rc) {
@@ -702,6 +724,11 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
         pi.in_elem_bytes = fmt_bytes(fmt_of(ps.src));
         pi.out_elem_bytes = fmt_bytes(fmt_of(ps.dst));
         pi.points = lines * ps.R;
+        pi.kernel = ps.small ? MFFT_KERNEL_STREAM
+                    : ps.fast_kind == KIND_ROWS ? MFFT_KERNEL_ROWS
+                    : ps.fast_kind == KIND_ROWS_TR ? MFFT_KERNEL_ROWS_TR
+                    : ps.fast_kind == KIND_COLS ? MFFT_KERNEL_COLS
+                    : ps.generic ? MFFT_KERNEL_GENERIC : MFFT_KERNEL_TILE;
     }
     info->out_scale = p->out_scale;
     info->workspace_bytes = p->work_bytes;

@@ -63,7 +63,10 @@ typedef struct mfft_desc {
     int in_fmt, out_fmt;      /* MFFT_FMT_* */
     int in_order;             /* MFFT_ORDER_* */
     int64_t workspace_limit;  /* bytes of intermediate buffer per plan (0 = default) */
-    int split_hint;           /* first-pass length for multi-pass 1D (0 = planner/autotuner) */
+    int split_hint;           /* first-pass length of a four-step split (0 = planner's choice;
+                                 1 = odd * 2^a lengths in one mixed-radix pass;
+                                 MFFT_SPLIT_TUNED = split and workspace_limit from the
+                                 autotuner's cache, planner's choice when absent) */
     int mid_fmt;              /* format of intermediates between passes: -1 = the precision's
                                  default (fp16 -> C16, bf16 -> CBF16, split/fp32 -> C32), or an
                                  MFFT_FMT_*; split fp16 with C16 rounds once per pass boundary */
@@ -84,7 +87,17 @@ typedef struct mfft_pass_info {
     int generic;    /* 1 = runtime-schedule kernel (odd radices) */
     int in_elem_bytes, out_elem_bytes; /* bytes per complex element read / written */
     int64_t points; /* complex elements read (= written) per launch of a full chunk */
+    int kernel;     /* MFFT_KERNEL_*: which kernel family runs the pass */
 } mfft_pass_info;
+
+enum {
+    MFFT_KERNEL_TILE = 0,      /* LDS tile kernel, power-of-two schedule (fft_tile_pow2) */
+    MFFT_KERNEL_GENERIC = 1,   /* LDS tile kernel, runtime mixed-radix schedule */
+    MFFT_KERNEL_ROWS = 2,      /* matrix-core kernel, contiguous lines in and out (fft_fast) */
+    MFFT_KERNEL_ROWS_TR = 3,   /* matrix-core kernel, contiguous in, transposed out (fft_fast) */
+    MFFT_KERNEL_COLS = 4,      /* matrix-core kernel, strided 256-point lines (fft_fast) */
+    MFFT_KERNEL_STREAM = 5     /* streaming short-DFT kernel, one thread per line (small_cols) */
+};
 
 typedef struct mfft_info {
     int kind;              /* 1 = 1D, 2 = 2D */
@@ -217,6 +230,32 @@ int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info);
 int mfft_conv2d_destroy(mfft_conv2d op);
 /* smallest transform length >= n that the planner supports well (2^a * {1,3,5,17}) */
 int mfft_good_size(int n);
+
+/* ------------------------------------------------------------------------------------
+ * Schedule autotuner. The reference fixes one schedule per size (tcfftCreate,
+ * tcfft_half.cu:1250-1330; tcfftCreate_2d, tcfft_half_2d.cu:1167-1206); mfft_autotune times
+ * the planner's candidates on the device -- four-step split point (split_hint) and batch
+ * chunk (workspace_limit); never the precision or mid_fmt -- and caches the fastest per
+ * (device arch, nx, ny, batch, precision, in/out/mid formats, direction, input order).
+ * Plans created with split_hint = MFFT_SPLIT_TUNED use the cached schedule. ny = 0 for 1D.
+ * The cache is process-wide; save/load keep it across processes (text, one entry a line).
+ * ------------------------------------------------------------------------------------ */
+#define MFFT_SPLIT_TUNED (-1)
+
+typedef struct mfft_tune_entry {
+    int split_hint;
+    int64_t workspace_limit;
+    double ms;           /* tuned schedule, ms per exec */
+    double default_ms;   /* the planner's default schedule, ms per exec (-1 if it did not run) */
+    int candidates;      /* schedules timed */
+} mfft_tune_entry;
+
+int mfft_autotune(int nx, int ny, int64_t batch, const mfft_desc *d, int reps, mfft_tune_entry *out);
+int mfft_tune_lookup(const char *arch, int nx, int ny, int64_t batch, const mfft_desc *d, mfft_tune_entry *out);
+int mfft_tune_cache_save(const char *path);
+int mfft_tune_cache_load(const char *path);
+int mfft_tune_cache_clear(void);
+const char *mfft_tune_last_error(void);
 
 /* ------------------------------------------------------------------------------------
  * Continuous-batching request server (no counterpart in the reference). Mixed 1D FFT,
