@@ -158,6 +158,7 @@ SIGNATURES.update({
     "mfft_conv2d_forward_select": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "mfft_conv2d_forward_two": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, _vp]),
+    "mfft_conv2d_backward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "mfft_conv2d_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(ConvInfo)]),
     "mfft_conv2d_destroy": (ctypes.c_int, [_vp]),
     "mfft_good_size": (ctypes.c_int, [ctypes.c_int]),

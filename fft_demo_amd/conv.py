@@ -87,6 +87,26 @@ class Conv2d:
                 "mfft_conv2d_forward_two")
         return out0, out1
 
+    def backward(self, delta, x=None, grad_in=True, grad_w=True, stream=None):
+        """Gradients of the 'valid_corr' operator: delta = dL/dout [B][F][oh][ow], x = the
+        forward input [B][C][h][w] (needed for grad_w). Returns (grad_in, grad_w), each
+        None when not asked for."""
+        import torch
+
+        d = self.desc
+        assert delta.is_cuda and delta.is_contiguous() and delta.numel() == d.batch * d.filters * self.out_shape[2] * \
+            self.out_shape[3]
+        gi = torch.empty((d.batch, d.channels, d.h, d.w), dtype=delta.dtype, device=delta.device) if grad_in else None
+        gw = torch.empty((d.filters, d.channels, d.fh, d.fw), dtype=delta.dtype, device=delta.device) \
+            if grad_w else None
+        if grad_w:
+            assert x is not None and x.is_cuda and x.is_contiguous() and x.dtype == delta.dtype
+        s = stream if stream is not None else torch.cuda.current_stream(delta.device)
+        ptr = (lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None)
+        L.check(L.load().mfft_conv2d_backward(self._h, ptr(x), ptr(delta), self._fmt(delta), ptr(gi), ptr(gw),
+                                               ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_backward")
+        return gi, gw
+
     def close(self):
         if getattr(self, "_h", None):
             L.load().mfft_conv2d_destroy(self._h)

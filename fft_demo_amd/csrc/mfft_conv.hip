@@ -139,6 +139,91 @@ __global__ void crop_two_kernel(const float2 *Y, T *out0, T *out1, int oh, int o
     out1[i] = (T)v.y;
 }
 
+// ---------------------------------------------------------------- backward
+// Filter gradient spectrum: G[f][c][p] = sum_k A_k[c] conj(DA_k[f]) + B_k[c] conj(DB_k[f]),
+// where X_k = spectrum of the image pair (a + i b) and D_k of the delta pair (da + i db),
+// separated per frequency with X(-p): A = (X + X*)/2, B = (X - X*)/2i (X* = conj X(-p)).
+// Its inverse is sum_b in[b][c] (correlated with) delta[b][f].
+__global__ void pointwise_gw_kernel(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph,
+                                    int Pw, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long long P = (long long)Ph * Pw;
+    const long long p = i % P;
+    const long long r = i / P;
+    const int c = (int)(r % C), f = (int)(r / C);
+    const int v = (int)(p % Pw), u = (int)(p / Pw);
+    const long long q = (long long)(u ? Ph - u : 0) * Pw + (v ? Pw - v : 0);
+    float gr = 0.f, gi = 0.f;
+    for (int k = 0; k < pairs; ++k) {
+        const float2 xp = X[((long long)k * C + c) * P + p], xq = X[((long long)k * C + c) * P + q];
+        const float2 dp = D[((long long)k * F + f) * P + p], dq = D[((long long)k * F + f) * P + q];
+        // A = (xp + conj xq)/2, B = (xp - conj xq)/(2i)
+        const float2 A = make_float2(0.5f * (xp.x + xq.x), 0.5f * (xp.y - xq.y));
+        const float2 B = make_float2(0.5f * (xp.y + xq.y), -0.5f * (xp.x - xq.x));
+        const float2 DA = make_float2(0.5f * (dp.x + dq.x), 0.5f * (dp.y - dq.y));
+        const float2 DB = make_float2(0.5f * (dp.y + dq.y), -0.5f * (dp.x - dq.x));
+        // A conj(DA) + B conj(DB)
+        gr += A.x * DA.x + A.y * DA.y + B.x * DB.x + B.y * DB.y;
+        gi += A.y * DA.x - A.x * DA.y + B.y * DB.x - B.x * DB.y;
+    }
+    G[i] = make_float2(gr, gi);
+}
+
+// Input gradient spectrum: R[k][c][p] = sum_f D_k[f][p] conj(H[f][c][p]) with H the spectrum
+// of the 180-degree rotated filter; conj(H) is the spectrum of a real signal, so the image
+// pair stays separable in the inverse.
+__global__ void pointwise_gi_kernel(const float2 *D, const float2 *H, float2 *R, int C, int F, long long P,
+                                    long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long long p = i % P;
+    const long long r = i / P;
+    const int c = (int)(r % C);
+    const long long k = r / C;
+    float re = 0.f, im = 0.f;
+    for (int f = 0; f < F; ++f) {
+        const float2 d = D[(k * F + f) * P + p];
+        const float2 h = H[((long long)f * C + c) * P + p];
+        re += d.x * h.x + d.y * h.y;
+        im += d.y * h.x - d.x * h.y;
+    }
+    R[i] = make_float2(re, im);
+}
+
+// out[b][c][y][x] = scale * (Re or Im) Y[b/2][c][(y + oy) mod Ph][(x + ox) mod Pw]
+template <typename T>
+__global__ void crop_cyclic_kernel(const float2 *Y, T *out, int C, int oh, int ow, int Ph, int Pw, int oy, int ox,
+                                   float scale, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % ow);
+    long long r = i / ow;
+    int y = (int)(r % oh);
+    r /= oh;
+    int c = (int)(r % C);
+    long long b = r / C;
+    const int yy = (y + oy) % Ph, xx = (x + ox) % Pw;
+    float2 v = Y[(((b >> 1) * C + c) * Ph + yy) * (long long)Pw + xx];
+    out[i] = (T)((b & 1 ? v.y : v.x) * scale);
+}
+
+// out[n][y][x] = scale * Re Y[n][y][x] for the top-left oh x ow window of each of N planes
+template <typename T>
+__global__ void crop_real_kernel(const float2 *Y, T *out, int oh, int ow, int Ph, int Pw, float scale, long long total)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int x = (int)(i % ow);
+    long long r = i / ow;
+    int y = (int)(r % oh);
+    long long n = r / oh;
+    out[i] = (T)(Y[(n * Ph + y) * (long long)Pw + x].x * scale);
+}
+
 static inline unsigned nblocks(long long n) { return (unsigned)((n + 255) / 256); }
 
 hipError_t conv_pack_pairs(const void *src, int f64, float2 *dst, int B, int C, int h, int w, int Ph, int Pw,
@@ -185,6 +270,48 @@ hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, 
     else
         hipLaunchKernelGGL(crop_unpack_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out, B, F, oh,
                            ow, Ph, Pw, oy, ox, scale, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_pointwise_gw(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph, int Pw,
+                             hipStream_t s)
+{
+    long long total = (long long)F * C * Ph * Pw;
+    hipLaunchKernelGGL(pointwise_gw_kernel, dim3(nblocks(total)), dim3(256), 0, s, X, D, G, pairs, C, F, Ph, Pw, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_pointwise_gi(const float2 *D, const float2 *H, float2 *R, long long pairs, int C, int F, long long P,
+                             hipStream_t s)
+{
+    long long total = pairs * C * P;
+    hipLaunchKernelGGL(pointwise_gi_kernel, dim3(nblocks(total)), dim3(256), 0, s, D, H, R, C, F, P, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_crop_cyclic(const float2 *Y, void *out, int f64, int B, int C, int oh, int ow, int Ph, int Pw, int oy,
+                            int ox, float scale, hipStream_t s)
+{
+    long long total = (long long)B * C * oh * ow;
+    if (f64)
+        hipLaunchKernelGGL(crop_cyclic_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out, C, oh,
+                           ow, Ph, Pw, oy, ox, scale, total);
+    else
+        hipLaunchKernelGGL(crop_cyclic_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out, C, oh,
+                           ow, Ph, Pw, oy, ox, scale, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_crop_real(const float2 *Y, void *out, int f64, long long planes, int oh, int ow, int Ph, int Pw,
+                          float scale, hipStream_t s)
+{
+    long long total = planes * oh * ow;
+    if (f64)
+        hipLaunchKernelGGL(crop_real_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out, oh, ow,
+                           Ph, Pw, scale, total);
+    else
+        hipLaunchKernelGGL(crop_real_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out, oh, ow,
+                           Ph, Pw, scale, total);
     return hipGetLastError();
 }
 

@@ -28,6 +28,14 @@ hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long
                           hipStream_t s);
 hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
                      float scale, hipStream_t s);
+hipError_t conv_pointwise_gw(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph, int Pw,
+                             hipStream_t s);
+hipError_t conv_pointwise_gi(const float2 *D, const float2 *H, float2 *R, long long pairs, int C, int F, long long P,
+                             hipStream_t s);
+hipError_t conv_crop_cyclic(const float2 *Y, void *out, int f64, int B, int C, int oh, int ow, int Ph, int Pw, int oy,
+                            int ox, float scale, hipStream_t s);
+hipError_t conv_crop_real(const float2 *Y, void *out, int f64, long long planes, int oh, int ow, int Ph, int Pw,
+                          float scale, hipStream_t s);
 hipError_t conv_pack_two(const void *a, const void *b, int f64, float2 *dst, int h, int w, int Ph, int Pw,
                          hipStream_t s);
 hipError_t conv_pointwise_two(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw,
@@ -42,6 +50,10 @@ struct mfft_conv2d_s {
     long long pairs = 0;
     mfft_plan fwd_img = nullptr, fwd_filt = nullptr, inv = nullptr;
     float2 *H = nullptr, *X = nullptr, *Y = nullptr;
+    // backward (built on its first call): delta transform, inverses over (filter, channel)
+    // and (pair, channel) batches, and the filter-gradient spectra
+    mfft_plan fwd_delta = nullptr, inv_fc = nullptr, inv_pc = nullptr;
+    float2 *G = nullptr;
     bool have_filters = false;
     long long bytes = 0, spec_bytes = 0;
     std::mutex mu;
@@ -110,6 +122,9 @@ int mfft_conv2d_destroy(mfft_conv2d op)
     if (op->fwd_img) mfft_plan_destroy(op->fwd_img);
     if (op->fwd_filt) mfft_plan_destroy(op->fwd_filt);
     if (op->inv) mfft_plan_destroy(op->inv);
+    for (mfft_plan p : {op->fwd_delta, op->inv_fc, op->inv_pc})
+        if (p) mfft_plan_destroy(p);
+    (void)hipFree(op->G);
     (void)hipFree(op->H);
     (void)hipFree(op->X);
     (void)hipFree(op->Y);
@@ -265,6 +280,85 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
     if (mfft::conv_crop_two(op->Y, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy, op->ox, s) !=
         hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
+    return MFFT_OK;
+}
+
+// Gradients of the 'valid' correlation layer (the reference's conv2d_backward,
+// tensorcore_fft_conv2d.cpp:364-442): with delta = dL/dout [B][F][oh][ow],
+//   grad_w[f][c]  = sum_b in[b][c] (correlated with) delta[b][f]   -> IFFT(X conj(D)), window [0,fh)x[0,fw)
+//   grad_in[b][c] = sum_f delta[b][f] (full conv) w[f][c]          -> IFFT(D conj(H_rot)), read at
+//                   the cyclic offset -(fh-1), -(fw-1) (H_rot: spectrum of the rotated filter)
+// on the forward's padded size (Ph >= h: neither product wraps into the window read).
+static int backward_setup(mfft_conv2d op)
+{
+    if (op->fwd_delta) return MFFT_OK;
+    const mfft_conv2d_desc &d = op->d;
+    mfft_desc fd;
+    mfft_desc_default(&fd);
+    fd.precision = d.precision;
+    fd.in_fmt = fd.out_fmt = MFFT_FMT_C32;
+    fd.workspace_limit = d.workspace_limit;
+    fd.scale = MFFT_SCALE_STAGE;
+    int rc = mfft_plan_create_2d(&op->fwd_delta, op->Ph, op->Pw, op->pairs * d.filters, &fd);
+    fd.scale = MFFT_SCALE_NONE;
+    fd.direction = MFFT_INVERSE;
+    if (!rc) rc = mfft_plan_create_2d(&op->inv_fc, op->Ph, op->Pw, (int64_t)d.filters * d.channels, &fd);
+    if (!rc) rc = mfft_plan_create_2d(&op->inv_pc, op->Ph, op->Pw, op->pairs * d.channels, &fd);
+    if (!rc && hipMalloc(&op->G, (size_t)d.filters * d.channels * op->Ph * op->Pw * 8) != hipSuccess) rc = MFFT_ERR_NOMEM;
+    if (rc) {
+        std::string m = mfft_last_error();
+        for (mfft_plan *p : {&op->fwd_delta, &op->inv_fc, &op->inv_pc})
+            if (*p) {
+                mfft_plan_destroy(*p);
+                *p = nullptr;
+            }
+        return cfail(rc, "backward setup: %s", m.c_str());
+    }
+    return MFFT_OK;
+}
+
+int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int real_fmt, void *grad_in, void *grad_w,
+                         void *stream)
+{
+    if (!op || !delta || (!grad_in && !grad_w) || (grad_w && !in)) return cfail(MFFT_ERR_INVALID, "null argument");
+    if (op->d.mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_UNSUPPORTED, "backward of the 'valid' correlation mode only");
+    if (grad_in && !op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
+    std::lock_guard<std::mutex> lk(op->mu);
+    int rc = backward_setup(op);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const mfft_conv2d_desc &d = op->d;
+    const int f64 = real_fmt == MFFT_REAL_F64;
+    const long long P = (long long)op->Ph * op->Pw;
+    // delta spectra (scaled 1/P) in Y
+    if (mfft::conv_pack_pairs(delta, f64, op->Y, d.batch, d.filters, op->oh, op->ow, op->Ph, op->Pw, s) != hipSuccess)
+        return cfail(MFFT_ERR_HIP, "pack delta");
+    rc = mfft_exec(op->fwd_delta, op->Y, op->Y, s);
+    if (rc) return cfail(rc, "%s", mfft_last_error());
+    if (grad_w) {
+        if (mfft::conv_pack_pairs(in, f64, op->X, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pack input");
+        rc = mfft_exec(op->fwd_img, op->X, op->X, s);
+        if (rc) return cfail(rc, "%s", mfft_last_error());
+        if (mfft::conv_pointwise_gw(op->X, op->Y, op->G, (int)op->pairs, d.channels, d.filters, op->Ph, op->Pw, s) !=
+            hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pointwise (filter gradient)");
+        rc = mfft_exec(op->inv_fc, op->G, op->G, s);
+        if (rc) return cfail(rc, "%s", mfft_last_error());
+        // both spectra carry 1/P and the unnormalised inverse one factor P back
+        if (mfft::conv_crop_real(op->G, grad_w, f64, (long long)d.filters * d.channels, d.fh, d.fw, op->Ph, op->Pw,
+                                 (float)P, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "crop (filter gradient)");
+    }
+    if (grad_in) {
+        if (mfft::conv_pointwise_gi(op->Y, op->H, op->X, op->pairs, d.channels, d.filters, P, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pointwise (input gradient)");
+        rc = mfft_exec(op->inv_pc, op->X, op->X, s);
+        if (rc) return cfail(rc, "%s", mfft_last_error());
+        if (mfft::conv_crop_cyclic(op->X, grad_in, f64, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw,
+                                   op->Ph - (d.fh - 1), op->Pw - (d.fw - 1), 1.0f, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "crop (input gradient)");
+    }
     return MFFT_OK;
 }
 

@@ -179,8 +179,8 @@ __device__ __forceinline__ void dft16(const float2 *x, const h8 &raw16, const b8
                 h8 bl;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    bl[2 * k] = (_Float16)(x[k].x - (float)bh[2 * k]);
-                    bl[2 * k + 1] = (_Float16)(x[k].y - (float)bh[2 * k + 1]);
+                    bl[2 * k] = (_Float16)__builtin_fmaf((float)bh[2 * k], -1.0f, x[k].x);
+                    bl[2 * k + 1] = (_Float16)__builtin_fmaf((float)bh[2 * k + 1], -1.0f, x[k].y);
                 }
                 ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hr, bl, ar, 0, 0, 0);
                 ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hi, bl, ai, 0, 0, 0);
@@ -242,13 +242,45 @@ struct Prefetch {
     }
 };
 
+// Four-step twiddles W_N^(mult * (pt0 + k * STEP)), k = 0..3, read before any store.
+template <int STEP>
+__device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsigned mult, int pt0, float2 *w)
+{
+    if (!a.tw_N) return;
+    const unsigned mask = (1u << a.tw_bits) - 1u;
+    // address-space-qualified scalar views: a load through them cannot become a flat load
+    typedef __attribute__((address_space(3))) const float lds_f;
+    typedef __attribute__((address_space(1))) const float glb_f;
+    if (a.tw_lds) {
+        const lds_f *t = (const lds_f *)twl;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned e = mult * (unsigned)(pt0 + k * STEP);
+            const unsigned ih = 2 * (kFastTw + (e >> a.tw_bits)), il = 2 * (e & mask);
+            w[k] = cmulf(make_float2(t[ih], t[ih + 1]), make_float2(t[il], t[il + 1]));
+        }
+    } else {
+        const glb_f *th = (const glb_f *)a.tw_hi, *tl = (const glb_f *)a.tw_lo_s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned e = mult * (unsigned)(pt0 + k * STEP);
+            const unsigned ih = 2 * (e >> a.tw_bits), il = 2 * (e & mask);
+            w[k] = cmulf(make_float2(th[ih], th[ih + 1]), make_float2(tl[il], tl[il + 1]));
+        }
+    }
+}
+
 // ------------------------------------------------------------------ radix-16 stage (MFMA)
 // FIRST: operands are the prefetched raw inputs; LAST: results go to global memory.
-// stw: stage twiddles (LDS or global, scale folded in); twlo/twhi: four-step tables.
+// stw: stage twiddles (LDS or global, scale folded in); twl: the four-step tables staged
+// in LDS (lo at [0, kFastTw), hi at [kFastTw, 2 kFastTw)) when a.tw_lds, else they are read
+// from global memory (a.tw_lo_s, a.tw_hi). The two sources are separate code paths so
+// each keeps its address space (LDS reads wait on the LDS counter only, never on the
+// next tile's prefetch loads).
 template <int PREC, int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF>
 __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Origin &o, int s, const Ops<PREC> &op,
                                        const Prefetch<PREC, R, KIND, INF> &pf, const float2 *stw,
-                                       const float2 *twlo, const float2 *twhi)
+                                       const float2 *twl)
 {
     typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
     constexpr int T = G::T, R16 = G::R16;
@@ -290,24 +322,26 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
         slot_tc<R, T, MAP>(P * 16 + c, t, col);
         if constexpr (!LAST) {
             const int q = col % s, p = col / s;
+            float2 w[4];  // all twiddles first: the reads do not wait behind the tile writes
+#pragma unroll
+            for (int i = 0; i < 4; ++i) w[i] = stw[p * (4 * g + i) * s];  // W_R^(p*tq*s), scale folded in
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int tq = 4 * g + i;
-                const float2 w = stw[p * tq * s];  // W_R^(p*tq*s), scale folded in
-                lds[G::phys(t, q + s * (16 * p + tq))] = cmulf(make_float2(accr[u][i], acci[u][i]), w);
+                lds[G::phys(t, q + s * (16 * p + tq))] = cmulf(make_float2(accr[u][i], acci[u][i]), w[i]);
             }
         } else {
             if (t < o.nv) {
                 typedef typename Elem<OUTF>::raw raw;
                 raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
-                const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+                float2 w[4];
+                tw4<R16>(a, twl, (unsigned)(a.tw_level ? o.l1 : o.l0b + t), col + R16 * 4 * g, w);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int pt = col + R16 * (4 * g + i);
                     float2 y = make_float2(accr[u][i], acci[u][i]);
                     if (a.tw_N) {
-                        const unsigned e = mult * (unsigned)pt;
-                        y = cmulf(y, cmulf(twhi[e >> a.tw_bits], twlo[e & ((1u << a.tw_bits) - 1u)]));
+                        y = cmulf(y, w[i]);
                     } else {
                         y.x *= a.last_mul;
                         y.y *= a.last_mul;
@@ -366,7 +400,7 @@ __device__ __forceinline__ void dft16v(float2 *v, int sign)
 template <int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF>
 __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const Origin &o, int s,
                                          const Prefetch<PREC_FP32, R, KIND, INF> &pf, const float2 *stw,
-                                         const float2 *twlo, const float2 *twhi)
+                                         const float2 *twl)
 {
     typedef Geo<R, KIND, fast_waves<PREC_FP32, KIND>()> G;
     constexpr int T = G::T, R16 = G::R16;
@@ -394,17 +428,22 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
         raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
         const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
 #pragma unroll
-        for (int tq = 0; tq < 16; ++tq) {
-            const int pt = col + R16 * tq;
-            float2 y = v[tq];
-            if (a.tw_N) {
-                const unsigned e = mult * (unsigned)pt;
-                y = cmulf(y, cmulf(twhi[e >> a.tw_bits], twlo[e & ((1u << a.tw_bits) - 1u)]));
-            } else {
-                y.x *= a.last_mul;
-                y.y *= a.last_mul;
+        for (int q4 = 0; q4 < 4; ++q4) {
+            float2 w[4];
+            tw4<4 * R16>(a, twl, mult, col + R16 * q4, w);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int tq = q4 + 4 * k;
+                const int pt = col + R16 * tq;
+                float2 y = v[tq];
+                if (a.tw_N) {
+                    y = cmulf(y, w[k]);
+                } else {
+                    y.x *= a.last_mul;
+                    y.y *= a.last_mul;
+                }
+                dst[(long long)pt * a.out_ps] = Elem<OUTF>::pack(y);
             }
-            dst[(long long)pt * a.out_ps] = Elem<OUTF>::pack(y);
         }
     }
 }
@@ -503,13 +542,10 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(Tile
         for (int i = threadIdx.x; i < R; i += G::NT) s_stw[i] = a.stage_tw_s[i];
         stw = s_stw;
     }
-    const float2 *twlo = a.tw_lo_s, *twhi = a.tw_hi;
     if (a.tw_N && a.tw_lds) {
         const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
         for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = a.tw_lo_s[i];
         for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[kFastTw + i] = a.tw_hi[i];
-        twlo = s_tw;
-        twhi = s_tw + kFastTw;
     }
     Ops<PREC> op;
     if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
@@ -542,9 +578,9 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(Tile
         }
         int s = 1;
         if constexpr (PREC == PREC_FP32)
-            vstage16<R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, cur, stw, twlo, twhi);
+            vstage16<R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
         else
-            mstage<PREC, R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, op, cur, stw, twlo, twhi);
+            mstage<PREC, R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
         s = 16;
         if constexpr (TAIL > 1) {
             tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul);
@@ -553,15 +589,15 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(Tile
 #pragma unroll
         for (int k = 1; k < NS - 1; ++k) {
             if constexpr (PREC == PREC_FP32)
-                vstage16<R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, cur, stw, twlo, twhi);
+                vstage16<R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
             else
-                mstage<PREC, R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, op, cur, stw, twlo, twhi);
+                mstage<PREC, R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
             s *= 16;
         }
         if constexpr (PREC == PREC_FP32)
-            vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, cur, stw, twlo, twhi);
+            vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
         else
-            mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, op, cur, stw, twlo, twhi);
+            mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
         __syncthreads();  // the next tile's first stage overwrites LDS
         tile = next;
         o = on;

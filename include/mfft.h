@@ -226,6 +226,15 @@ int mfft_conv2d_forward_select(mfft_conv2d op, const void *in, int in_fmt, void 
  * nothing). */
 int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, int in_fmt, void *out0, void *out1,
                             int out_fmt, int filter0, int filter1, void *stream);
+/* Gradients of the 'valid' correlation operator (MFFT_CONV_VALID_CORR; the reference's
+ * fft_conv2d_half::conv2d_backward, tensorcore_fft_conv2d.cpp:364-442). delta = dL/dout,
+ * [B][F][out_h][out_w]; in = the forward input [B][C][h][w] (needed for grad_w).
+ *   grad_in [B][C][h][w]  = sum_f delta[b][f] 'full'-convolved with filter[f][c]
+ *   grad_w  [F][C][fh][fw] = sum_b in[b][c] 'valid'-correlated with delta[b][f]
+ * Either output may be NULL. All real buffers in real_fmt. The first call builds the
+ * backward plans and a filter-gradient buffer (allocation); later calls allocate nothing. */
+int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int real_fmt, void *grad_in, void *grad_w,
+                         void *stream);
 int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info);
 int mfft_conv2d_destroy(mfft_conv2d op);
 /* smallest transform length >= n that the planner supports well (2^a * {1,3,5,17}) */

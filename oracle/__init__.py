@@ -8,6 +8,7 @@ Each function follows the reference (yuelingfan/fft_demo):
   gen_rev            tcfft_half.cu:1351-1374
   fft / fft2d        main.cpp:68-94 (FFTW FORWARD convention, unnormalised)
   corr2d_valid       main.cpp:97-116
+  conv2d_full        tensorcore_fft_conv2d.cpp:400-442 (input gradient of the 'valid' layer)
   err_capped         main.cpp:118-134
   fft_batch_loop     main.cpp:68-80 (the reference's CPU harness shape)
 """
@@ -55,6 +56,7 @@ def _load():
     lib.oracle_fft_batch_loop.restype = ctypes.c_int
     lib.oracle_corr2d_valid.argtypes = [_dp, _dp, _dp] + [ctypes.c_int] * 4
     lib.oracle_conv2d_same.argtypes = [_dp, _dp, _dp] + [ctypes.c_int] * 7
+    lib.oracle_conv2d_full.argtypes = [_dp, _dp, _dp] + [ctypes.c_int] * 4
     lib.oracle_err_capped.argtypes = [_dp, _dp, _i64, _i64]
     lib.oracle_err_capped.restype = ctypes.c_double
     lib.oracle_err_l2max.argtypes = [_dp, _dp, _i64, _dp, _dp]
@@ -162,6 +164,18 @@ def corr2d_valid(signal: np.ndarray, filt: np.ndarray) -> np.ndarray:
     fy, fx = f.shape
     out = np.empty((sy - fy + 1, sx - fx + 1), dtype=np.float64)
     _load().oracle_corr2d_valid(_ptr(s), _ptr(f), _ptr(out), sy, sx, fy, fx)
+    return out
+
+
+def conv2d_full(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Direct FP64 'full' convolution (the input gradient of the 'valid' correlation layer,
+    tensorcore_fft_conv2d.cpp:400-442)."""
+    x = _f64(a)
+    y = _f64(b)
+    ah, aw = x.shape
+    bh, bw = y.shape
+    out = np.empty((ah + bh - 1, aw + bw - 1), dtype=np.float64)
+    _load().oracle_conv2d_full(_ptr(x), _ptr(y), _ptr(out), ah, aw, bh, bw)
     return out
 
 

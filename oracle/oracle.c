@@ -353,6 +353,25 @@ void oracle_corr2d_valid(const double *signal, const double *filter, double *out
         }
 }
 
+void oracle_conv2d_full(const double *a, const double *b, double *out, int ah, int aw, int bh, int bw)
+{
+    const int oh = ah + bh - 1, ow = aw + bw - 1;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < oh; ++y)
+        for (int x = 0; x < ow; ++x) {
+            double acc = 0.0;
+            for (int i = 0; i < bh; ++i) {
+                const int ay = y - i;
+                if (ay < 0 || ay >= ah) continue;
+                for (int j = 0; j < bw; ++j) {
+                    const int ax = x - j;
+                    if (ax >= 0 && ax < aw) acc += a[ay * aw + ax] * b[i * bw + j];
+                }
+            }
+            out[y * ow + x] = acc;
+        }
+}
+
 void oracle_conv2d_same(const double *image, const double *filter, double *out, int h, int w,
                         int fh, int fw, int row0, int row1, int threads)
 {

@@ -62,6 +62,11 @@ void oracle_corr2d_valid(const double *signal, const double *filter, double *out
 /* 'same' 2D convolution (true convolution, filter flipped), output h x w, centred on the
  * full convolution with offset ((fh-1)/2, (fw-1)/2), zero boundary. Rows [row0, row1)
  * only (pass 0, h for the whole image). */
+/* Direct FP64 'full' 2D convolution out[Y][X] = sum_{i,j} a[Y-i][X-j] * b[i][j],
+ * out of size (ah+bh-1) x (aw+bw-1). The input gradient of the 'valid' correlation layer
+ * (the quantity tensorcore_fft_conv2d.cpp:400-442 computes for conv2d_backward). */
+void oracle_conv2d_full(const double *a, const double *b, double *out, int ah, int aw, int bh, int bw);
+
 void oracle_conv2d_same(const double *image, const double *filter, double *out, int h, int w,
                         int fh, int fw, int row0, int row1, int threads);
 

@@ -114,3 +114,64 @@ def test_conv_forward_two_separates_paired_images(dev, mode):
             mx = np.abs(got - ref).max() / np.abs(ref).max()
             print(f"forward_two {mode} f={f} rel_l2={l2:.2e} max={mx:.2e}")
             assert l2 <= 1e-4 and mx <= 1e-3
+
+
+def _backward_case(dev, B, C, F, h, w, fh, fw, dtype, seed):
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    oh, ow = h - fh + 1, w - fw + 1
+    x = oracle.uniform(B * C * h * w, seed, 0.0, 1.0).reshape(B, C, h, w)
+    filt = oracle.uniform(F * C * fh * fw, seed + 1, -1.0, 1.0).reshape(F, C, fh, fw)
+    delta = oracle.uniform(B * F * oh * ow, seed + 2, -1.0, 1.0).reshape(B, F, oh, ow)
+    td = getattr(torch, dtype)
+    op = Conv2d(B, C, F, h, w, fh, fw, mode="valid_corr")
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=td).contiguous())
+    xd = torch.from_numpy(x).to(device=dev, dtype=td).contiguous()
+    dd = torch.from_numpy(delta).to(device=dev, dtype=td).contiguous()
+    y = op.forward(xd)
+    gi, gw = op.backward(dd, xd)
+    torch.cuda.synchronize()
+    return x, filt, delta, y.double().cpu().numpy(), gi.double().cpu().numpy(), gw.double().cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_conv_backward_matches_direct_gradients(dev, dtype):
+    """Gradients of the 'valid' correlation layer (the reference's conv2d_backward,
+    tensorcore_fft_conv2d.cpp:364-442) against the direct FP64 definitions: grad_w =
+    sum_b corr_valid(in[b][c], delta[b][f]); grad_in = sum_f conv_full(delta[b][f], w[f][c]).
+    Odd batch (the last image pair has an empty half). Tolerance rel-L2 <= 1e-4 per map."""
+    B, C, F, h, w, fh, fw = 3, 2, 2, 40, 52, 5, 7
+    x, filt, delta, _, gi, gw = _backward_case(dev, B, C, F, h, w, fh, fw, dtype, 61)
+    for f in range(F):
+        for c in range(C):
+            ref = sum(oracle.corr2d_valid(x[b, c], delta[b, f]) for b in range(B))
+            l2 = np.linalg.norm(gw[f, c] - ref) / np.linalg.norm(ref)
+            print(f"grad_w f={f} c={c} rel_l2={l2:.2e}")
+            assert l2 <= 1e-4
+    for b in range(B):
+        for c in range(C):
+            ref = sum(oracle.conv2d_full(delta[b, f], filt[f, c]) for f in range(F))
+            l2 = np.linalg.norm(gi[b, c] - ref) / np.linalg.norm(ref)
+            print(f"grad_in b={b} c={c} rel_l2={l2:.2e}")
+            assert l2 <= 1e-4
+
+
+@pytest.mark.gpu
+def test_conv_backward_adjoint_identities_large(dev):
+    """Size-independent check at 1024 x 1024 with a 31 x 31 filter (pad 1024 = the image):
+    <forward(x), delta> = <x, grad_in(delta)> = <w, grad_w(x, delta)>. The inner products
+    cancel heavily, so the bound is 1e-4 of the Cauchy-Schwarz scale ||a|| ||b|| of each
+    product (what a relative L2 error of 1e-4 in any one operand can move it by)."""
+    B, C, F, h, w, k = 2, 1, 1, 1024, 1024, 31
+    x, filt, delta, y, gi, gw = _backward_case(dev, B, C, F, h, w, k, k, "float32", 71)
+    lhs = float(np.sum(y * delta))
+    r_in = float(np.sum(x * gi))
+    r_w = float(np.sum(filt * gw))
+    scale = max(np.linalg.norm(y) * np.linalg.norm(delta), np.linalg.norm(x) * np.linalg.norm(gi),
+                np.linalg.norm(filt) * np.linalg.norm(gw))
+    print(f"<y,d>={lhs:.6e} <x,gi>={r_in:.6e} <w,gw>={r_w:.6e} scale={scale:.3e}")
+    assert abs(r_in - lhs) <= 1e-4 * scale
+    assert abs(r_w - lhs) <= 1e-4 * scale

@@ -191,6 +191,31 @@ def test_conv_same_is_flipped_correlation():
     assert np.abs(oracle.conv2d_same(img, f) - want).max() < 1e-12
 
 
+def test_conv_full_against_scipy_bruteforce_and_adjoint():
+    """'full' convolution: scipy.signal.convolve2d (independent library), a brute-force
+    sum, and the adjoint identity of the 'valid' correlation layer that makes it the
+    input gradient: <corr_valid(s, w), d> = <s, conv_full(d, w)> and the filter gradient
+    <corr_valid(s, w), d> = <w, corr_valid(s, d)>."""
+    from scipy import signal
+
+    a = cplx((7, 9), 11).real
+    b = cplx((3, 4), 12).real
+    y = oracle.conv2d_full(a, b)
+    assert y.shape == (9, 12)
+    assert np.abs(y - signal.convolve2d(a, b, mode="full")).max() < 1e-12
+    want = np.zeros((9, 12))
+    for i in range(3):
+        for j in range(4):
+            want[i:i + 7, j:j + 9] += b[i, j] * a
+    assert np.abs(y - want).max() < 1e-12
+    s = cplx((13, 10), 13).real
+    w = cplx((4, 3), 14).real
+    d = cplx((10, 8), 15).real  # shape of corr_valid(s, w)
+    lhs = np.sum(oracle.corr2d_valid(s, w) * d)
+    assert abs(lhs - np.sum(s * oracle.conv2d_full(d, w))) < 1e-10
+    assert abs(lhs - np.sum(w * oracle.corr2d_valid(s, d))) < 1e-10
+
+
 # ---------------------------------------------------------------- metrics
 def test_err_capped_semantics():
     """main.cpp:118-134: sum of min(1,|(t-s)/s|) over re and im, / (n * batch)."""
