@@ -243,6 +243,28 @@ def _cpu_model():
     return "unknown"
 
 
+def copy_bandwidth(mib=1024, reps=10):
+    """Practical HBM ceiling on this device: a device-to-device copy of `mib` MiB, read +
+    write bytes per second (GB/s)."""
+    import torch
+
+    a = torch.ones((mib << 20) // 4, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2 * (mib << 20) / (ms * 1e-3) / 1e9, 1)
+
+
 def profiled_traffic():
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy
     kernel of known size), from the summaries committed under profiles/: {key: bytes}, key =
@@ -536,6 +558,8 @@ def main():
         "algorithmic_bytes_per_launch": algo_bytes,
         "avg_launch_ms": round(pp[k], 5),
     }
+    roofline["copy_gbs"] = copy_bandwidth()
+    roofline["frac_of_copy"] = round(achieved / roofline["copy_gbs"], 4)
     n = math.prod(shape)
     whole_bytes = batch * n * 8  # fp16 complex in + out
     line = {

@@ -41,5 +41,20 @@ int main(int argc, char **argv)
     }
     printf("conv s=%d f=%d F=%d pad=%d capped=%.3e rel_l2=%.3e\n", s, f, F, layer.padded_x, capped / ((double)o * o * F),
            worst_l2);
-    return worst_l2 <= 1e-4 ? 0 : 1;
+    // backward (tensorcore_fft_conv2d.cpp:364-442): dL/dout = the forward output itself;
+    // filter gradient of filter 0 against the direct 'valid' correlation signal * dL/dout
+    layer.conv2d_backward(out.data());
+    double num = 0, den = 0;
+    for (int a = 0; a < f; ++a)
+        for (int b = 0; b < f; ++b) {
+            double acc = 0;
+            for (int i = 0; i < o; ++i)
+                for (int j = 0; j < o; ++j) acc += signal[(size_t)(i + a) * s + j + b] * out[(size_t)i * o + j];
+            double t = layer.delta_for_filter[(size_t)a * f + b];
+            num += (t - acc) * (t - acc);
+            den += acc * acc;
+        }
+    const double bw_l2 = std::sqrt(num / den);
+    printf("conv_backward filter0 rel_l2=%.3e\n", bw_l2);
+    return worst_l2 <= 1e-4 && bw_l2 <= 1e-4 ? 0 : 1;
 }

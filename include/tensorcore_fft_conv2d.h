@@ -16,11 +16,16 @@
  *   conv2d_forward(in, w, out, S, F, sy, sx, fy, fx, GPU)  .cpp:291-363
  *   fft_2d_c2c / ifft_2d_c2c                     .cpp:79-214   host FP64 complex batches
  *                                                     over num_streams streams
+ *   conv2d_backward(double *delta_in)             .cpp:364-442  host FP64 dL/dout in;
+ *                                                     delta_for_filter [F][S][fy][fx] and
+ *                                                     delta_for_signal [S][sy][sx] out
  * Differences by design: any sizes (the reference pads only to 256 or 512), fp32 spectra
- * from split-fp16 transforms instead of fp16 spectra with power-of-10 quantisation, and no
+ * from split-fp16 transforms instead of fp16 spectra with power-of-10 quantisation, no
  * double free of the device buffers (the reference frees them in fft_pointwise and again
- * in the destructor, _kernel.cu:1110-1111 / .cpp:73-74). The backward pass
- * (conv2d_backward) is not provided.
+ * in the destructor, _kernel.cu:1110-1111 / .cpp:73-74), and the backward pass computes
+ * the gradients of the 'valid' correlation exactly (input gradient = 'full' convolution
+ * of dL/dout with the filters, filter gradient = 'valid' correlation of the signal with
+ * dL/dout) on the forward's padded size, with the output arrays owned by the object.
  */
 #ifndef TENSORCORE_FFT_CONV2D_H_MI355X
 #define TENSORCORE_FFT_CONV2D_H_MI355X
@@ -84,6 +89,29 @@ class fft_conv2d_half {
         conv2d_forward(out);
     }
 
+    /* dL/dout (F maps of output_y x output_x doubles, host) -> delta_for_filter and
+     * delta_for_signal (host arrays owned by this object) */
+    double *delta_for_filter = nullptr, *delta_for_signal = nullptr;
+    void conv2d_backward(double *delta_in)
+    {
+        const size_t nd = (size_t)F * output_x * output_y, ng = (size_t)F * S * fx * fy, ns = (size_t)S * sx * sy;
+        void *d_delta = nullptr, *d_gw = nullptr, *d_gi = nullptr;
+        check_hip(hipMalloc(&d_delta, nd * sizeof(double)));
+        check_hip(hipMalloc(&d_gw, ng * sizeof(double)));
+        check_hip(hipMalloc(&d_gi, ns * sizeof(double)));
+        check_hip(hipMemcpy(d_delta, delta_in, nd * sizeof(double), hipMemcpyHostToDevice));
+        check(mfft_conv2d_backward(op_, d_signal_, d_delta, MFFT_REAL_F64, d_gi, d_gw, nullptr), "conv2d_backward");
+        grad_w_.resize(ng);
+        grad_in_.resize(ns);
+        check_hip(hipMemcpy(grad_w_.data(), d_gw, ng * sizeof(double), hipMemcpyDeviceToHost));
+        check_hip(hipMemcpy(grad_in_.data(), d_gi, ns * sizeof(double), hipMemcpyDeviceToHost));
+        (void)hipFree(d_delta);
+        (void)hipFree(d_gw);
+        (void)hipFree(d_gi);
+        delta_for_filter = grad_w_.data();
+        delta_for_signal = grad_in_.data();
+    }
+
     void TensorCoreFFTCreate(fftPlan *plan, int nx, int ny, int n_batch, int fft, __GPU_ARCH__ gpu)
     {
         mfft_desc d;
@@ -127,6 +155,7 @@ class fft_conv2d_half {
   private:
     mfft_conv2d op_ = nullptr;
     void *d_signal_ = nullptr, *d_out_ = nullptr;
+    std::vector<double> grad_w_, grad_in_;
     std::vector<mfft_plan> plans_;
 
     static void check(int rc, const char *what)
