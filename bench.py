@@ -2,8 +2,9 @@
 
 Default workload (the metric's configuration): batched 1D complex FFT, N=65536 (radix
 16^4, two 256-point passes), batch=256, fp16 interleaved input and output, split-fp16
-(hi+lo) matrix-core stages with fp32 accumulation and per-stage 1/16 scaling. One step =
-one exec of the whole batch with the input already resident in HBM.
+(hi+lo) matrix-core stages with fp32 accumulation and per-stage 1/16 scaling, fp16
+intermediates between the two passes (the fp32-intermediate variant is reported beside
+it). One step = one exec of the whole batch with the input already resident in HBM.
 
 Printed (one JSON line): GFLOP/s = 5*N*log2(N)*batch / t, rel-L2 and rel-max vs the FP64
 oracle on a sample of the batch, the plain fp32 GPU FFT (this library's fp32 VALU path)
@@ -38,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FFT GFLOP/s (5NlogN) at rel-L2≤1e-3 vs plain FP32 GPU FFT; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+TUNE = True
 
 CONFIGS = {
     # name: (shape, batch, description)
@@ -132,17 +134,17 @@ def accuracy(plan, xh, out, sample, direction=-1):
     return oracle.err_l2max(yc, ref)
 
 
-def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample):
+def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt=None):
     import torch
 
     import fft_demo_amd as F
-
     from fft_demo_amd import tune
 
     prec = name
-    # every variant gets the same treatment: autotuned schedule (split point, batch chunk)
-    sched = tune.autotune(shape, batch, precision=prec)
-    plan = F.Plan(shape, batch, precision=prec, split_hint=tune.SPLIT_TUNED)
+    # every variant gets the same treatment: autotuned schedule (split point, batch chunk),
+    # unless --no-tune (profiling runs: only the planner's schedule is launched)
+    sched = tune.autotune(shape, batch, precision=prec, mid_fmt=mid_fmt) if TUNE else None
+    plan = F.Plan(shape, batch, precision=prec, split_hint=tune.SPLIT_TUNED if TUNE else 0, mid_fmt=mid_fmt)
     dt = plan.in_dtype()
     x = torch.from_numpy(xh).to(device=dev, dtype=dt).contiguous()
     out = plan.empty_out(device=dev)
@@ -481,12 +483,15 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="trace length of the serving run (c5: 30)")
     ap.add_argument("--serve-p99-ms", type=float, default=2.0, help="p99 bound of the equal-p99 comparison")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="planner's schedule, no autotuning (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="headline precision only")
     args = ap.parse_args()
+    global TUNE
+    TUNE = not args.no_tune
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched FFT shards along "
               f"the batch with no exchange, so the per-GPU figure is the 1-GPU figure.", file=sys.stderr)
@@ -525,9 +530,13 @@ def main():
     xh = make_input(shape, batch)
     sample = 4 if len(shape) == 1 else 1
 
-    head = run_variant("split_fp16", shape, batch, xh, args.steps, args.warmup, dev, stream, sample)
+    # headline: split fp16 with fp16 intermediates between passes (one rounding per pass
+    # boundary; rel-L2 stays within 1e-3); the fp32-intermediate variant is a comparator
+    head = run_variant("split_fp16", shape, batch, xh, args.steps, args.warmup, dev, stream, sample, mid_fmt="c16")
     comps = {}
     if not args.no_compare:
+        comps["split_fp16_c32_intermediates"] = run_variant("split_fp16", shape, batch, xh, args.steps, args.warmup,
+                                                            dev, stream, sample)
         for name in ("fp32", "fp16", "bf16"):
             comps[name] = run_variant(name, shape, batch, xh, args.steps, args.warmup, dev, stream, sample)
         try:
@@ -576,7 +585,8 @@ def main():
         "dtype": "fp16",
         "data": "synthetic: re/im uniform[-1,1] from SplitMix64 seed 42, rounded to fp16",
         "config": {"workload": desc, "shape": list(shape), "batch": batch, "precision": "split_fp16 (hi+lo, fp32 acc)",
-                   "io": "c16 in / c16 out", "scale": "1/16 per radix-16 stage (out = DFT/N)"},
+                   "io": "c16 in / c16 out", "intermediates": "c16 between passes",
+                   "scale": "1/16 per radix-16 stage (out = DFT/N)"},
         "rel_l2": head["rel_l2"],
         "rel_max": head["rel_max"],
         "accuracy_sample": f"{sample} of {batch} transforms vs FP64 oracle",
