@@ -346,7 +346,8 @@ def _rocfft_plain(trace, filters, dev):
 
 
 def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, log):
-    """Highest arrival rate (x1.5 steps from start_rate) whose p99 stays within the bound;
+    """Highest arrival rate (x1.5 steps from start_rate, then two bisection steps) whose
+    p99 stays within the bound;
     returns the replay result at that rate. shapes_trace covers the request shapes (plans
     and operators are built from it before the probes)."""
     from fft_demo_amd import serve
@@ -357,15 +358,29 @@ def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, 
     serve.register_filters(S, filters)
     ar = serve.Arena(S, shapes_trace, 16 << 30)
     serve.warm(S, shapes_trace, ar)
+    def probe(rate):
+        r = serve.replay(S, serve.make_trace(rate, seconds, 43), ar)
+        log.append({"rate": round(rate, 1), "p99_ms": round(r["p99_ms"], 3), "req_per_s": round(r["req_per_s"], 1)})
+        return r, r["p99_ms"] <= target_p99_ms and r["req_per_s"] >= 0.9 * rate
+
     try:
+        fail = None
         for _ in range(10):
-            tr = serve.make_trace(rate, seconds, 43)
-            r = serve.replay(S, tr, ar)
-            log.append({"rate": round(rate, 1), "p99_ms": round(r["p99_ms"], 3), "req_per_s": round(r["req_per_s"], 1)})
-            if r["p99_ms"] > target_p99_ms or r["req_per_s"] < 0.9 * rate:
+            r, ok = probe(rate)
+            if not ok:
+                fail = rate
                 break
-            best = r
+            best, lo = r, rate
             rate *= 1.5
+        # two geometric bisection steps between the last rate that held and the first that did not
+        if best is not None and fail is not None:
+            for _ in range(2):
+                mid = (lo * fail) ** 0.5
+                r, ok = probe(mid)
+                if ok:
+                    best, lo = r, mid
+                else:
+                    fail = mid
     finally:
         S.close()
         del ar

@@ -130,6 +130,12 @@ mfft_plan get_plan(mfft_server_s *S, int kind, int nx, int ny, int dir)
     int rc = kind == MFFT_REQ_FFT1D ? mfft_plan_create_1d(&p, nx, S->d.max_batch_1d, &d)
                                     : mfft_plan_create_2d(&p, nx, ny, S->d.max_batch_2d, &d);
     if (rc) return nullptr;
+    mfft_info inf;
+    if (mfft_plan_info(p, &inf) || inf.workspace_bytes > S->slab_bytes) {
+        // one transform's intermediates do not fit a stream's slab
+        mfft_plan_destroy(p);
+        return nullptr;
+    }
     S->plans[key] = p;
     return p;
 }
@@ -284,7 +290,7 @@ int launch_one(mfft_server_s *S, Req *q, int st)
     if (r.kind == MFFT_REQ_FFT1D || r.kind == MFFT_REQ_FFT2D) {
         mfft_plan p = r.kind == MFFT_REQ_FFT1D ? get_plan(S, r.kind, r.n, 0, r.direction)
                                                : get_plan(S, r.kind, r.nx, r.ny, r.direction);
-        if (!p) return sfail(MFFT_ERR_UNSUPPORTED, "no plan");
+        if (!p) return sfail(MFFT_ERR_UNSUPPORTED, "no plan for this shape within the stream workspace (%lld bytes): %s", S->slab_bytes, mfft_last_error());
         mfft_info inf;
         mfft_plan_info(p, &inf);
         q->out_scale = inf.out_scale;
