@@ -4,7 +4,7 @@
 // path = ctor (tensorcore_fft_conv2d.cpp:12-58: pad, digit-reverse, FFT signal and
 // filter), TENSORCORE_FFT_CONV_2D (:250-273: pointwise, inverse FFT, dequantise) and
 // readout_fwd (:275-282: crop at fx-1, scale 1/(Nx*Ny)). Differences by design: any image
-// size (padding to 2^a*{1,3,5,17} instead of 256/512 only, :23-32), 'same' convolution
+// size (padding to 2^a*{1,3,5,9,15,17} instead of 256/512 only, :23-32), 'same' convolution
 // besides the reference's 'valid' correlation, batches with pairs of real images sharing
 // a complex transform, fp32 spectra with per-stage-scaled split-fp16 transforms instead of
 // fp16 spectra with per-filter power-of-10 quantisation, resident filter spectra reused
@@ -74,14 +74,14 @@ int cfail(int code, const char *fmt, ...)
     return code;
 }
 
+// Padded sizes odd * 2^a whose odd part has a streaming pass of its own (the planner then
+// runs the power of two on the matrix-core kernels); products such as 51 = 3 * 17 would
+// fall back to the mixed-radix tile kernel.
 bool good(int n)
 {
     if (n < 2 || n > 4096 * 256) return false;
     while (n % 2 == 0) n /= 2;
-    const int odd[] = {17, 5, 3};
-    for (int p : odd)
-        if (n % p == 0) n /= p;  // at most one odd factor keeps the schedule short
-    return n == 1;
+    return n == 1 || n == 3 || n == 5 || n == 9 || n == 15 || n == 17;
 }
 
 int plan_info_bytes(mfft_plan p)

@@ -237,7 +237,7 @@ int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int 
                          void *stream);
 int mfft_conv2d_info_get(mfft_conv2d op, mfft_conv2d_info *info);
 int mfft_conv2d_destroy(mfft_conv2d op);
-/* smallest transform length >= n that the planner supports well (2^a * {1,3,5,17}) */
+/* smallest transform length >= n that runs on the fast path: 2^a * {1, 3, 5, 9, 15, 17} */
 int mfft_good_size(int n);
 
 /* ------------------------------------------------------------------------------------
