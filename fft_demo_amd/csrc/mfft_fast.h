@@ -1,4 +1,5 @@
-// Fast tile kernels of the matrix-core FFT (power-of-two line lengths >= 256).
+// Fast tile kernels of the matrix-core FFT (power-of-two line lengths 64..4096; the
+// strided kind 64..256).
 //
 // Same method as the generic tile kernel (radix-16 DFT-matrix products on MFMA, Stockham
 // schedule, LDS between stages) with the HBM traffic moved into the stages themselves:
@@ -24,6 +25,6 @@ constexpr int kColsT = 32;  // lines per tile of the strided kind
 typedef void (*fast_fn)(TileArgs);
 fast_fn find_fast_kernel(int prec, int R, int kind, int in_fmt, int out_fmt);
 int fast_tile_lines(int R, int kind);
-int fast_threads(int prec, int kind);
+int fast_threads(int prec, int R, int kind);
 
 }  // namespace mfft

@@ -56,9 +56,16 @@ struct Elem<FMT_C32> {
 };
 
 // ------------------------------------------------------------------ geometry
-// Waves per workgroup: the strided kind holds kColsT lines per tile (twice the points).
-template <int PREC, int KIND>
-constexpr int fast_waves() { return (PREC == PREC_FP32 ? 4 : 8) * (KIND == KIND_COLS ? kColsT / 16 : 1); }
+// Waves per workgroup. A tile is T lines of R points (T = kMaxTile / R; kColsT for the
+// strided kind). fp32: one 16-point column per thread in the radix-16 stages. Matrix-core
+// precisions: two 16-column MFMA groups per wave, 4..16 waves.
+template <int PREC, int KIND, int R>
+constexpr int fast_waves()
+{
+    constexpr int T = KIND == KIND_COLS ? kColsT : kMaxTile / R;
+    constexpr int groups = R * T / 256;
+    return PREC == PREC_FP32 ? R * T / 16 / 64 : (groups / 2 < 4 ? 4 : (groups / 2 > 16 ? 16 : groups / 2));
+}
 
 template <int R, int KIND, int NW_ = 4>
 struct Geo {
@@ -196,7 +203,7 @@ __device__ __forceinline__ void dft16(const float2 *x, const h8 &raw16, const b8
 // tile's loads can be in flight while the current tile is transformed.
 template <int PREC, int R, int KIND, int INF>
 struct Prefetch {
-    typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
     typedef typename Elem<INF>::raw raw;
     static constexpr int N = PREC == PREC_FP32 ? 16 : 4 * G::MAXP;
     raw w[N];
@@ -243,7 +250,7 @@ struct Prefetch {
 };
 
 // Four-step twiddles W_N^(mult * (pt0 + k * STEP)), k = 0..3, read before any store.
-template <int STEP>
+template <int STEP, bool SCALED = true>
 __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsigned mult, int pt0, float2 *w)
 {
     if (!a.tw_N) return;
@@ -260,7 +267,7 @@ __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsign
             w[k] = cmulf(make_float2(t[ih], t[ih + 1]), make_float2(t[il], t[il + 1]));
         }
     } else {
-        const glb_f *th = (const glb_f *)a.tw_hi, *tl = (const glb_f *)a.tw_lo_s;
+        const glb_f *th = (const glb_f *)a.tw_hi, *tl = (const glb_f *)(SCALED ? a.tw_lo_s : a.tw_lo);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const unsigned e = mult * (unsigned)(pt0 + k * STEP);
@@ -282,7 +289,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
                                        const Prefetch<PREC, R, KIND, INF> &pf, const float2 *stw,
                                        const float2 *twl)
 {
-    typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
     constexpr int T = G::T, R16 = G::R16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -402,7 +409,7 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
                                          const Prefetch<PREC_FP32, R, KIND, INF> &pf, const float2 *stw,
                                          const float2 *twl)
 {
-    typedef Geo<R, KIND, fast_waves<PREC_FP32, KIND>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC_FP32, KIND, R>()> G;
     constexpr int T = G::T, R16 = G::R16;
     static_assert(R * G::T / 16 == G::NT, "one column per thread");
     int t, col;
@@ -518,21 +525,73 @@ __device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, fl
     __syncthreads();
 }
 
+// The radix-2/4/8 tail as the LAST stage (R = 16 * TAIL: one matrix-core stage, then
+// this): reads LDS, and stores straight to global memory with the four-step twiddle (the
+// unscaled tables: this stage is VALU, no x16 operand factor), the tail's stage scale and
+// the output conversion. Lanes walk the points of a line (ROWS) or the lines (the
+// transposed and strided kinds), whichever is contiguous in the output.
+template <int R, int KIND, int RADIX, int NW, int OUTF>
+__device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, const Origin &o, const float2 *twl)
+{
+    typedef Geo<R, KIND, NW> G;
+    constexpr int T = G::T;
+    constexpr int COLS = R / RADIX;  // = 16: the stride of the last stage
+    constexpr int ITEMS = COLS * T;
+    constexpr int NT = G::NT;
+    constexpr int PER = (ITEMS + NT - 1) / NT;
+    typedef typename Elem<OUTF>::raw raw;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int I = threadIdx.x + k * NT;
+        if (ITEMS % NT != 0 && I >= ITEMS) break;
+        const int t = KIND == KIND_ROWS ? I / COLS : I % T;
+        const int col = KIND == KIND_ROWS ? I % COLS : I / T;
+        float2 v[RADIX];
+#pragma unroll
+        for (int j = 0; j < RADIX; ++j) v[j] = lds[G::phys(t, col + j * COLS)];
+        if (t >= o.nv) continue;
+        dft_tail<RADIX>(v, a.sign);
+        raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
+        const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+#pragma unroll
+        for (int q4 = 0; q4 < RADIX; q4 += 4) {
+            float2 w[4];
+            if constexpr (RADIX >= 4)
+                tw4<COLS, false>(a, twl, mult, col + COLS * q4, w);
+#pragma unroll
+            for (int kk = 0; kk < (RADIX < 4 ? RADIX : 4); ++kk) {
+                const int tq = q4 + kk;
+                const int pt = col + COLS * tq;
+                float2 y = make_float2(v[tq].x * a.tail_mul, v[tq].y * a.tail_mul);
+                if (a.tw_N) {
+                    if constexpr (RADIX >= 4) {
+                        y = cmulf(y, w[kk]);
+                    } else {
+                        const unsigned e = mult * (unsigned)pt, mask = (1u << a.tw_bits) - 1u;
+                        y = cmulf(y, cmulf(a.tw_hi[e >> a.tw_bits], a.tw_lo[e & mask]));
+                    }
+                }
+                dst[(long long)pt * a.out_ps] = Elem<OUTF>::pack(y);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ kernel
 // R = 16^n * TAIL (TAIL in {1,2,4,8}, n >= 2): schedule [16, TAIL?, 16, ..., 16].
 // Persistent: workgroup b transforms tiles b, b + gridDim.x, ...; the first-stage
 // operands of the next tile are loaded before the current tile is transformed.
 template <int PREC, int R, int KIND, int INF, int OUTF>
-__global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(TileArgs a)
+__global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(TileArgs a)
 {
-    typedef Geo<R, KIND, fast_waves<PREC, KIND>()> G;
+    typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
     __shared__ float2 lds[G::SIZE];
     __shared__ float2 s_stw[G::STW > 0 ? G::STW : 1];
     __shared__ float2 s_tw[2 * kFastTw];
     constexpr int L2 = __builtin_ctz(R);
     constexpr int NS = L2 / 4;
     constexpr int TAIL = 1 << (L2 % 4);
-    static_assert(NS >= 2, "fast kernels need R >= 256");
+    static_assert(NS >= 2 || (NS == 1 && TAIL > 1), "fast kernels need R >= 32");
     constexpr int MAP_IN = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
     constexpr int MAP_OUT = KIND == KIND_ROWS ? MAP_POINTS : MAP_LINES;
 
@@ -544,7 +603,9 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(Tile
     }
     if (a.tw_N && a.tw_lds) {
         const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
-        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = a.tw_lo_s[i];
+        // the tail-as-last stage (NS == 1) is VALU: it takes the unscaled lo table
+        const float2 *lo = NS == 1 ? a.tw_lo : a.tw_lo_s;
+        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = lo[i];
         for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[kFastTw + i] = a.tw_hi[i];
     }
     Ops<PREC> op;
@@ -582,22 +643,27 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(Tile
         else
             mstage<PREC, R, KIND, MAP_IN, true, false, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
         s = 16;
-        if constexpr (TAIL > 1) {
-            tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul);
-            s *= TAIL;
-        }
+        if constexpr (NS == 1) {
+            // R = 16 * TAIL: the tail is the last stage and stores to global memory
+            tstage_last<R, KIND, TAIL, G::NW, OUTF>(lds, a, o, s_tw);
+        } else {
+            if constexpr (TAIL > 1) {
+                tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul);
+                s *= TAIL;
+            }
 #pragma unroll
-        for (int k = 1; k < NS - 1; ++k) {
+            for (int k = 1; k < NS - 1; ++k) {
+                if constexpr (PREC == PREC_FP32)
+                    vstage16<R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
+                else
+                    mstage<PREC, R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
+                s *= 16;
+            }
             if constexpr (PREC == PREC_FP32)
-                vstage16<R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
+                vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
             else
-                mstage<PREC, R, KIND, MAP_IN, false, false, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
-            s *= 16;
+                mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
         }
-        if constexpr (PREC == PREC_FP32)
-            vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
-        else
-            mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
         __syncthreads();  // the next tile's first stage overwrites LDS
         tile = next;
         o = on;
@@ -608,18 +674,22 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND>() * 64)) fft_fast(Tile
 template <int PREC, int KIND, int INF, int OUTF>
 fast_fn pick_r(int R)
 {
-    if constexpr (KIND == KIND_COLS) {
-        return R == 256 ? fft_fast<PREC, 256, KIND, INF, OUTF> : nullptr;
-    } else {
+    switch (R) {
+    case 64: return fft_fast<PREC, 64, KIND, INF, OUTF>;
+    case 128: return fft_fast<PREC, 128, KIND, INF, OUTF>;
+    case 256: return fft_fast<PREC, 256, KIND, INF, OUTF>;
+    default: break;
+    }
+    if constexpr (KIND != KIND_COLS) {
         switch (R) {
-        case 256: return fft_fast<PREC, 256, KIND, INF, OUTF>;
         case 512: return fft_fast<PREC, 512, KIND, INF, OUTF>;
         case 1024: return fft_fast<PREC, 1024, KIND, INF, OUTF>;
         case 2048: return fft_fast<PREC, 2048, KIND, INF, OUTF>;
         case 4096: return fft_fast<PREC, 4096, KIND, INF, OUTF>;
-        default: return nullptr;
+        default: break;
         }
     }
+    return nullptr;
 }
 
 template <int PREC, int INF, int OUTF>

@@ -324,8 +324,8 @@ static int finish_passes(mfft_plan_s *p)
                 ps.radix.assign(1, ps.R);
             }
         }
-        // fast kernel: power of two 256..4096, no fused permutation
-        if (!ps.small && pow2 && ps.R >= 256 && ps.R <= kMaxTile && ps.perm_axis < 0) {
+        // fast kernel: power of two 64..4096, no fused permutation
+        if (!ps.small && pow2 && ps.R >= 64 && ps.R <= kMaxTile && ps.perm_axis < 0) {
             int kind = -1, l0 = -1;
             int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
             int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
@@ -337,7 +337,7 @@ static int finish_passes(mfft_plan_s *p)
             } else if (ps.ps_in == 1 && unit_out >= 0) {
                 kind = KIND_ROWS_TR;
                 l0 = unit_out;
-            } else if (ps.ps_in != 1 && ps.R == 256) {
+            } else if (ps.ps_in != 1 && ps.R <= 256) {
                 for (int i = 0; i < (int)ps.lines.size(); ++i)
                     if (ps.lines[i].in_stride == 1 && ps.lines[i].out_stride == 1) { l0 = i; break; }
                 if (l0 >= 0) kind = KIND_COLS;
@@ -352,7 +352,7 @@ static int finish_passes(mfft_plan_s *p)
                 int dev = 0, ncu = 0, nb = 0;
                 HIP_TRY(hipGetDevice(&dev));
                 HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, fast_threads(p->d.precision, kind), 0));
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, fast_threads(p->d.precision, ps.R, kind), 0));
                 ps.max_grid = (long long)std::max(1, nb) * std::max(1, ncu);
                 ps.max_grid -= ps.max_grid % 8;
                 if (ps.max_grid < 8) ps.max_grid = 8;
@@ -564,7 +564,7 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             long long grid = std::min(ntiles, ps.max_grid);
             if (grid > 8) grid -= grid % 8;
             void *args[] = {&a};
-            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)grid), dim3(fast_threads(p->d.precision, ps.fast_kind)), args, 0, s));
+            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)grid), dim3(fast_threads(p->d.precision, ps.R, ps.fast_kind)), args, 0, s));
         } else {
             HIP_TRY(launch_tile(ps.fn, a, ntiles, s));
         }
