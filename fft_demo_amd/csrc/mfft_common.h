@@ -66,6 +66,18 @@ struct TileArgs {
     float last_mul;            // last-stage scale when there is no pass twiddle
     float tail_mul;            // scale of the radix-2/4/8 stage
     int tw_lds;                // four-step tables fit in LDS (2^tw_bits and the hi part <= kFastTw)
+    // fused first-pass input (streaming kernel only; zero/null otherwise), element e of the
+    // chunk = transform batch_base + e / real_p, offset e % real_p inside it:
+    //   in_mul: the loaded value is multiplied by in_mul[e % in_mul_period]
+    //   real_src: the value is (img[2k][c][y][x], img[2k+1][c][y][x]) of real images
+    //   [real_b][real_c][real_h][real_w] (k = transform / real_c, c = transform % real_c;
+    //   zero outside the image: the padding of the FFT convolution)
+    const float2 *in_mul;
+    long long in_mul_period;
+    const void *real_src;
+    const void *real_src2;     // set: the imaginary part comes from this second image instead
+    int real_f64, real_b, real_c, real_h, real_w, real_pw;
+    long long real_p, batch_base;
 };
 
 // DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of

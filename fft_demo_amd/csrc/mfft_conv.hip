@@ -76,7 +76,8 @@ __global__ void pointwise_mac_kernel(const float2 *X, const float2 *H, float2 *Y
     Y[i] = make_float2(re, im);
 }
 
-// out[b][f][y][x] = scale * (Re or Im) Y[b/2][f][y + oy][x + ox]
+// out[2k][f][y][x] = scale * Re Y[k][f][y + oy][x + ox], out[2k+1][f][y][x] = scale * Im ...
+// (one thread per pair element: each spectrum value is read once for both images)
 template <typename T>
 __global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
                                    int ox, float scale, long long total)
@@ -88,9 +89,11 @@ __global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh
     int y = (int)(r % oh);
     r /= oh;
     int f = (int)(r % F);
-    long long b = r / F;
-    float2 v = Y[(((b >> 1) * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox)];
-    out[i] = (T)((b & 1 ? v.y : v.x) * scale);
+    long long k = r / F;
+    const float2 v = Y[((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox)];
+    const long long plane = (long long)F * oh * ow, off = ((long long)f * oh + y) * ow + x;
+    out[2 * k * plane + off] = (T)(v.x * scale);
+    if (2 * k + 1 < B) out[(2 * k + 1) * plane + off] = (T)(v.y * scale);
 }
 
 // Two real images a, b (separate buffers, [h][w]) -> one complex image a + i b, zero-padded.
@@ -263,7 +266,7 @@ hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long
 hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
                      float scale, hipStream_t s)
 {
-    long long total = (long long)B * F * oh * ow;
+    long long total = (long long)((B + 1) / 2) * F * oh * ow;
     if (f64)
         hipLaunchKernelGGL(crop_unpack_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out, B, F,
                            oh, ow, Ph, Pw, oy, ox, scale, total);

@@ -10,6 +10,7 @@
 // fp16 spectra with per-filter power-of-10 quantisation, resident filter spectra reused
 // across calls, and no per-call allocation.
 #include "../../include/mfft.h"
+#include "mfft_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -226,17 +227,38 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
     std::lock_guard<std::mutex> lk(op->mu);
     hipStream_t s = (hipStream_t)stream;
     const mfft_conv2d_desc &d = op->d;
-    if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw, s) !=
-        hipSuccess)
-        return cfail(MFFT_ERR_HIP, "pack");
-    int rc = mfft_exec(op->fwd_img, op->X, op->X, s);
+    int rc;
+    if (mfft::plan_first_pass_fusable(op->fwd_img)) {
+        // the forward transform's first pass reads the real images, zero-padded
+        mfft::ExecFuse f;
+        f.real_src = in;
+        f.real_f64 = in_fmt == MFFT_REAL_F64;
+        f.real_b = d.batch;
+        f.real_c = d.channels;
+        f.real_h = d.h;
+        f.real_w = d.w;
+        rc = mfft::exec_fused(op->fwd_img, op->X, op->X, op->pairs * d.channels, s, f);
+    } else {
+        if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw,
+                                  s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pack");
+        rc = mfft_exec(op->fwd_img, op->X, op->X, s);
+    }
     if (rc) return cfail(rc, "%s", mfft_last_error());
     const long long P = (long long)op->Ph * op->Pw;
     const int nf = fsel >= 0 ? 1 : d.filters;
     const float2 *H = fsel >= 0 ? op->H + (long long)fsel * d.channels * P : op->H;
-    if (mfft::conv_pointwise(op->X, H, op->Y, op->pairs, d.channels, nf, P, s) != hipSuccess)
-        return cfail(MFFT_ERR_HIP, "pointwise");
-    rc = mfft_exec_ex(op->inv, op->Y, op->Y, op->pairs * nf, nullptr, s);
+    if (d.channels == 1 && nf == 1 && mfft::plan_first_pass_fusable(op->inv)) {
+        // one filter, one channel: the inverse's first pass multiplies by the spectrum
+        mfft::ExecFuse f;
+        f.in_mul = H;
+        f.in_mul_period = P;
+        rc = mfft::exec_fused(op->inv, op->X, op->Y, op->pairs, s, f);
+    } else {
+        if (mfft::conv_pointwise(op->X, H, op->Y, op->pairs, d.channels, nf, P, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pointwise");
+        rc = mfft_exec_ex(op->inv, op->Y, op->Y, op->pairs * nf, nullptr, s);
+    }
     if (rc) return cfail(rc, "%s", mfft_last_error());
     if (mfft::conv_crop(op->Y, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph, op->Pw, op->oy,
                         op->ox, 1.0f, s) != hipSuccess)
@@ -268,9 +290,22 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
     std::lock_guard<std::mutex> lk(op->mu);
     hipStream_t s = (hipStream_t)stream;
     const long long P = (long long)op->Ph * op->Pw;
-    if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, d.h, d.w, op->Ph, op->Pw, s) != hipSuccess)
-        return cfail(MFFT_ERR_HIP, "pack");
-    int rc = mfft_exec_ex(op->fwd_img, op->X, op->X, 1, nullptr, s);
+    int rc;
+    if (mfft::plan_first_pass_fusable(op->fwd_img)) {
+        mfft::ExecFuse f;  // the two images read straight into the first pass
+        f.real_src = in0;
+        f.real_src2 = in1;
+        f.real_f64 = in_fmt == MFFT_REAL_F64;
+        f.real_b = 2;
+        f.real_c = 1;
+        f.real_h = d.h;
+        f.real_w = d.w;
+        rc = mfft::exec_fused(op->fwd_img, op->X, op->X, 1, s, f);
+    } else {
+        if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, d.h, d.w, op->Ph, op->Pw, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pack");
+        rc = mfft_exec_ex(op->fwd_img, op->X, op->X, 1, nullptr, s);
+    }
     if (rc) return cfail(rc, "%s", mfft_last_error());
     if (mfft::conv_pointwise_two(op->X, op->H + filter0 * P, op->H + filter1 * P, op->Y, op->Ph, op->Pw, s) !=
         hipSuccess)

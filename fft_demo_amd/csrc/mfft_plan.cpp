@@ -11,6 +11,7 @@
 #include "../../include/mfft.h"
 #include "mfft_common.h"
 #include "mfft_fast.h"
+#include "mfft_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -501,11 +502,25 @@ static void *buf_ptr(int which, const void *in, void *out, void *work)
 }
 
 static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s, hipEvent_t *ev,
-                      int &evi, void *work)
+                      int &evi, void *work, const mfft::ExecFuse *fz = nullptr, long long b0 = 0)
 {
     for (auto &ps : p->passes) {
         TileArgs a;
         memset(&a, 0, sizeof a);
+        if (fz && &ps == &p->passes.front()) {
+            a.in_mul = fz->in_mul;
+            a.in_mul_period = fz->in_mul_period;
+            a.real_src = fz->real_src;
+            a.real_src2 = fz->real_src2;
+            a.real_f64 = fz->real_f64;
+            a.real_b = fz->real_b;
+            a.real_c = fz->real_c;
+            a.real_h = fz->real_h;
+            a.real_w = fz->real_w;
+            a.real_pw = p->kind == 2 ? p->ny : p->n;
+            a.real_p = p->total;
+            a.batch_base = b0;
+        }
         a.in = buf_ptr(ps.src, in, out, work);
         a.out = buf_ptr(ps.dst, in, out, work);
         a.in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
@@ -738,7 +753,7 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
 }
 
 static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEvent_t *ev, long long batch = -1,
-                    void *work = nullptr)
+                    void *work = nullptr, const mfft::ExecFuse *fz = nullptr)
 {
     std::unique_lock<std::mutex> lk(p->exec_mu, std::defer_lock);
     if (!work) lk.lock();  // the plan's own workspace is shared by its callers
@@ -749,7 +764,8 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
     int evi = 0;
     for (long long b0 = 0; b0 < batch; b0 += p->chunk) {
         long long nb = std::min(p->chunk, batch - b0);
-        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work);
+        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work,
+                            fz, b0);
         if (rc) return rc;
     }
     if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
@@ -989,3 +1005,19 @@ const char *mfft_last_error(void) { return g_err.c_str(); }
 const char *mfft_version(void) { return "mfft 0.1 (gfx950 matrix-core FFT)"; }
 
 }  // extern "C"
+
+namespace mfft {
+
+bool plan_first_pass_fusable(mfft_plan p)
+{
+    return p && !p->passes.empty() && p->passes.front().small && p->passes.front().src == BUF_IN;
+}
+
+int exec_fused(mfft_plan p, const void *in, void *out, int64_t batch, void *stream, const ExecFuse &f)
+{
+    if (!plan_first_pass_fusable(p)) return fail(MFFT_ERR_UNSUPPORTED, "first pass takes no fused input");
+    if (batch < 1 || batch > p->batch) return fail(MFFT_ERR_INVALID, "batch %lld outside 1..%lld", (long long)batch, p->batch);
+    return exec_all(p, in, out, stream, nullptr, batch, nullptr, &f);
+}
+
+}  // namespace mfft

@@ -146,8 +146,42 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
     const long long ib = (long long)l0 * a.in_s0 + (long long)l1 * a.in_s1 + l2 * a.in_s2;
     const long long ob = (long long)l0 * a.out_s0 + (long long)l1 * a.out_s1 + l2 * a.out_s2;
     float2 v[R];
+    if (a.real_src) {
+        // conv: two real images of a pair, zero-padded (replaces a separate packing pass)
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = ld<INF>(a.in, ib + (long long)j * a.in_ps);
+        for (int j = 0; j < R; ++j) {
+            const long long e = ib + (long long)j * a.in_ps;
+            const long long tr = e / a.real_p;
+            const long long rem = e - tr * a.real_p;
+            const int y = (int)(rem / a.real_pw), x = (int)(rem - (long long)y * a.real_pw);
+            const long long t = a.batch_base + tr;
+            const long long k = t / a.real_c, c = t - k * a.real_c;
+            float2 val = make_float2(0.f, 0.f);
+            if (y < a.real_h && x < a.real_w) {
+                const long long hw = (long long)a.real_h * a.real_w, off = (long long)y * a.real_w + x;
+                const long long i0 = (2 * k * a.real_c + c) * hw + off, i1 = ((2 * k + 1) * a.real_c + c) * hw + off;
+                const bool second = a.real_src2 || 2 * k + 1 < a.real_b;
+                const void *src1 = a.real_src2 ? a.real_src2 : a.real_src;
+                const long long j1 = a.real_src2 ? i0 : i1;
+                if (a.real_f64) {
+                    val.x = (float)reinterpret_cast<const double *>(a.real_src)[i0];
+                    if (second) val.y = (float)reinterpret_cast<const double *>(src1)[j1];
+                } else {
+                    val.x = reinterpret_cast<const float *>(a.real_src)[i0];
+                    if (second) val.y = reinterpret_cast<const float *>(src1)[j1];
+                }
+            }
+            v[j] = val;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = ld<INF>(a.in, ib + (long long)j * a.in_ps);
+    }
+    if (a.in_mul) {
+        // conv: the pointwise product with the resident filter spectrum, fused into the load
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = cm(v[j], a.in_mul[(ib + (long long)j * a.in_ps) % a.in_mul_period]);
+    }
     if constexpr (R & 1)
         dft_odd<R>(v, a.stage_tw);
     else
