@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "FFT GFLOP/s (5NlogN) at rel-L2≤1e-3 vs plain FP32 GPU FFT; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 TUNE = True
+TIMING = ""
 
 CONFIGS = {
     # name: (shape, batch, description)
@@ -65,10 +66,11 @@ def make_input(shape, batch, seed=42):
 
 
 def time_plan(plan, x, out, steps, warmup, stream):
-    """Time `steps` back-to-back execs between two HIP events on the exec stream (the
-    figure reported); then a second run of `steps` execs with an event between every
-    launch gives each pass's average launch time (for the roofline). Returns
-    (total_ms of the first run, [per-pass avg ms])."""
+    """Time `steps` back-to-back execs between two HIP events on the exec stream; when an
+    exec is shorter than 0.5 ms (the Python wrapper's launch overhead would then leave gaps
+    in the queue) the figure reported is the replay of a hipGraph holding the `steps` execs.
+    A second run of `steps` execs with an event between every launch gives each pass's
+    average launch time (for the roofline). Returns (total_ms, [per-pass avg ms])."""
     import torch
 
     from fft_demo_amd.hiptime import EventPool
@@ -86,9 +88,12 @@ def time_plan(plan, x, out, steps, warmup, stream):
     ends.record(1, stream.cuda_stream)
     torch.cuda.synchronize()
     total = ends.elapsed_ms(0, 1)
-    if total / steps < 0.05:
-        # launch-bound (a few microseconds per exec): capture the steps in one hipGraph and
-        # time its replay, so host launch overhead is not what is measured
+    global TIMING
+    TIMING = "back-to-back execs between two HIP events"
+    if total / steps < 0.5:
+        # short execs (host launch overhead of the Python wrapper is comparable to the GPU
+        # time): capture the steps in one hipGraph and time its replay
+        TIMING = "hipGraph replay of the steps (captured execs), between two HIP events"
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             cs = torch.cuda.current_stream()  # the capture stream
@@ -161,6 +166,7 @@ def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_
         "batch_chunk": plan.info["batch_chunk"],
         "launches_per_exec": plan.info["launches_per_exec"],
         "schedule": sched,
+        "timing": TIMING,
     }
     del x, out
     plan.close()
@@ -644,6 +650,7 @@ def main():
         "pass_avg_ms": [round(v, 5) for v in pp],
         "batch_chunk": head["batch_chunk"],
         "schedule": head["schedule"],
+        "timing": head["timing"],
         "comparators": comps,
         "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3) if "fp32" in comps else None,
         "roofline": roofline,

@@ -25,6 +25,11 @@
 #include <tuple>
 #include <vector>
 
+namespace mfft {
+hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s,
+                       double scale = 1.0);
+}
+
 namespace {
 
 thread_local std::string t_err;
@@ -152,11 +157,32 @@ int mfft_autotune(int nx, int ny, int64_t batch, const mfft_desc *d, int reps, m
     hipStream_t s = nullptr;
     void *in = nullptr, *outb = nullptr;
     const size_t ib = (size_t)n * batch * fmt_bytes(d->in_fmt), ob = (size_t)n * batch * fmt_bytes(d->out_fmt);
-    if (hipMalloc(&in, ib) != hipSuccess || hipMalloc(&outb, ob) != hipSuccess ||
-        hipMemset(in, 0, ib) != hipSuccess || hipStreamCreate(&s) != hipSuccess) {
+    if (hipMalloc(&in, ib) != hipSuccess || hipMalloc(&outb, ob) != hipSuccess || hipStreamCreate(&s) != hipSuccess) {
         (void)hipFree(in);
         (void)hipFree(outb);
         return tfail(MFFT_ERR_NOMEM, "tuning buffers (%zu + %zu bytes)", ib, ob);
+    }
+    // representative input (all-zero data runs measurably faster): points of the unit circle
+    // W_1009^(37 i), generated on the device and converted to the plan's input format
+    {
+        bool ok = true;
+        if (d->in_fmt == MFFT_FMT_C32) {
+            ok = mfft::gen_twiddle((float2 *)in, 1009, n * batch, 37, -1, s) == hipSuccess;
+        } else {
+            void *tmp = nullptr;
+            ok = hipMalloc(&tmp, (size_t)n * batch * 8) == hipSuccess &&
+                 mfft::gen_twiddle((float2 *)tmp, 1009, n * batch, 37, -1, s) == hipSuccess &&
+                 mfft_stage_convert(tmp, MFFT_FMT_C32, in, d->in_fmt, n * batch, 1, nullptr, 0, nullptr, 0, 1.0, s) ==
+                     MFFT_OK &&
+                 hipStreamSynchronize(s) == hipSuccess;
+            (void)hipFree(tmp);
+        }
+        if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+            (void)hipStreamDestroy(s);
+            (void)hipFree(in);
+            (void)hipFree(outb);
+            return tfail(MFFT_ERR_HIP, "tuning input");
+        }
     }
     Entry best;
     best.ms = 1e30;

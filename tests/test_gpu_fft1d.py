@@ -103,3 +103,17 @@ def test_mixed_radix_single_pass(dev, n):
     assert len(plan.info["passes"]) == 1
     l2, mx = rel_errors(y, ref)
     assert l2 <= 1e-3 and mx <= 4e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp16", "bf16", "fp32"])
+@pytest.mark.parametrize("n", [64, 128, 8192, 1 << 14])
+def test_short_line_kernels_all_precisions(dev, n, prec):
+    """64- and 128-point passes (one radix-16 stage plus a radix-4/8 tail on the matrix-core
+    kernels): single-pass 64/128 and the four-step 8192 = 128 x 64, 16384 = 128 x 128."""
+    b = max(1, min(64, (1 << 18) // n))
+    y, ref, plan = _run(dev, n, b, prec, seed=n % 97 + 3)
+    l2, mx = rel_errors(y, ref)
+    kinds = [p["kernel"] for p in plan.info["passes"]]
+    print(f"N={n} b={b} {prec}: rel_l2={l2:.3e} rel_max={mx:.3e} passes={[p['R'] for p in plan.info['passes']]} {kinds}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]

@@ -67,3 +67,17 @@ def test_2d_inverse_and_fp32(dev):
         y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
         l2, _ = rel_errors(y, oracle.fft2d(x, sign=1))
         assert l2 <= tol, (prec, l2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fp16", 4e-3), ("bf16", 3e-2), ("fp32", 1e-5)])
+def test_2d_short_axes_other_precisions(dev, prec, tol):
+    """2D with 64/128-point axes (rows on the contiguous kind, columns on the strided kind)."""
+    fmt = {"fp16": "c16", "bf16": "cbf16", "fp32": "c32"}[prec]
+    for nx, ny in [(64, 128), (128, 64), (2176, 128)]:
+        plan = _plan(nx, ny, 2, precision=prec)
+        x = complex_input((2, nx, ny), nx * 3 + ny, fmt)
+        y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
+        l2, _ = rel_errors(y, oracle.fft2d(x))
+        print(f"2D {nx}x{ny} {prec}: rel_l2={l2:.2e} kernels={[p['kernel'] for p in plan.info['passes']]}")
+        assert l2 <= tol, (nx, ny, prec, l2)
