@@ -95,11 +95,9 @@ struct mfft_server_s {
     std::atomic<long long> used{0};  // request I/O of admitted, unfinished requests
     long long peak = 0;              // largest static_bytes + used seen at an admission
     long long submitted = 0, completed = 0, launches = 0, coalesced = 0, paired = 0;
-    double busy_since = 0;
     std::atomic<bool> stop{false};
     int device = 0;
     std::thread sched;
-    std::string error;
 };
 
 namespace {
