@@ -87,4 +87,33 @@ struct alignas(16) DftOps {
     __bf16 br[64][8], bi[64][8];                          // bf16 real and imag rows
 };
 
+// ------------------------------------------------------------------ fp16 / bf16 packing
+typedef float pk_f2 __attribute__((ext_vector_type(2)));
+typedef _Float16 pk_h2 __attribute__((ext_vector_type(2)));
+typedef __bf16 pk_b2 __attribute__((ext_vector_type(2)));
+
+// (re, im) -> one word of two fp16 / bf16 in one packed conversion (round to nearest even,
+// the same result as two scalar conversions).
+__device__ __forceinline__ uint32_t pack_c16(float re, float im)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((pk_f2){re, im}, pk_h2));
+}
+__device__ __forceinline__ uint32_t pack_cbf16(float re, float im)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((pk_f2){re, im}, pk_b2));
+}
+
+// Split fp32 (re, im) into fp16 pairs hi = fp16(x), lo = fp16(x - hi); x - hi is exact in
+// fp32. The difference is formed as fma(hi, m, x) with m = -1 hidden from the optimiser by
+// an empty asm statement, so it compiles to v_fma_mix_f32 reading the fp16 half in place
+// instead of a conversion back to fp32 and a subtraction.
+__device__ __forceinline__ void split_c16(float re, float im, pk_h2 &hi, pk_h2 &lo)
+{
+    hi = __builtin_convertvector((pk_f2){re, im}, pk_h2);
+    float m = -1.0f;
+    __asm__("" : "+s"(m));
+    lo = __builtin_convertvector((pk_f2){__builtin_fmaf((float)hi.x, m, re), __builtin_fmaf((float)hi.y, m, im)},
+                                 pk_h2);
+}
+
 }  // namespace mfft

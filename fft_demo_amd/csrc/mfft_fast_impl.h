@@ -29,11 +29,7 @@ struct Elem<FMT_C16> {
         _Float16 im = __builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
         return make_float2((float)re, (float)im);
     }
-    static __device__ __forceinline__ raw pack(float2 v)
-    {
-        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.x) |
-               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.y) << 16);
-    }
+    static __device__ __forceinline__ raw pack(float2 v) { return pack_c16(v.x, v.y); }
 };
 template <>
 struct Elem<FMT_CBF16> {
@@ -42,11 +38,7 @@ struct Elem<FMT_CBF16> {
     {
         return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
     }
-    static __device__ __forceinline__ raw pack(float2 v)
-    {
-        return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.x) |
-               ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.y) << 16);
-    }
+    static __device__ __forceinline__ raw pack(float2 v) { return pack_cbf16(v.x, v.y); }
 };
 template <>
 struct Elem<FMT_C32> {
@@ -169,26 +161,28 @@ __device__ __forceinline__ void dft16(const float2 *x, const h8 &raw16, const b8
         ar = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op.br, b, ar, 0, 0, 0);
         ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op.bi, b, ai, 0, 0, 0);
     } else {
-        h8 bh;
+        h8 bh, bl;
         if constexpr (EXACT16) {
             bh = raw16;
         } else {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                bh[2 * k] = (_Float16)x[k].x;
-                bh[2 * k + 1] = (_Float16)x[k].y;
+                pk_h2 hv, lv;
+                if constexpr (PREC == PREC_SPLIT) {
+                    split_c16(x[k].x, x[k].y, hv, lv);
+                    bl[2 * k] = lv.x;
+                    bl[2 * k + 1] = lv.y;
+                } else {
+                    hv = __builtin_convertvector((pk_f2){x[k].x, x[k].y}, pk_h2);
+                }
+                bh[2 * k] = hv.x;
+                bh[2 * k + 1] = hv.y;
             }
         }
         if constexpr (PREC == PREC_SPLIT) {
             ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.lr, bh, ar, 0, 0, 0);
             ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.li, bh, ai, 0, 0, 0);
             if constexpr (!EXACT16) {
-                h8 bl;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    bl[2 * k] = (_Float16)__builtin_fmaf((float)bh[2 * k], -1.0f, x[k].x);
-                    bl[2 * k + 1] = (_Float16)__builtin_fmaf((float)bh[2 * k + 1], -1.0f, x[k].y);
-                }
                 ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hr, bl, ar, 0, 0, 0);
                 ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(op.hi, bl, ai, 0, 0, 0);
             }
@@ -253,7 +247,6 @@ struct Prefetch {
 template <int STEP, bool SCALED = true>
 __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsigned mult, int pt0, float2 *w)
 {
-    if (!a.tw_N) return;
     const unsigned mask = (1u << a.tw_bits) - 1u;
     // address-space-qualified scalar views: a load through them cannot become a flat load
     typedef __attribute__((address_space(3))) const float lds_f;
@@ -278,13 +271,14 @@ __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsign
 }
 
 // ------------------------------------------------------------------ radix-16 stage (MFMA)
-// FIRST: operands are the prefetched raw inputs; LAST: results go to global memory.
+// FIRST: operands are the prefetched raw inputs; LAST: results go to global memory,
+// multiplied by the four-step twiddles when TW (a.tw_N != 0), else by a.last_mul.
 // stw: stage twiddles (LDS or global, scale folded in); twl: the four-step tables staged
 // in LDS (lo at [0, kFastTw), hi at [kFastTw, 2 kFastTw)) when a.tw_lds, else they are read
 // from global memory (a.tw_lo_s, a.tw_hi). The two sources are separate code paths so
 // each keeps its address space (LDS reads wait on the LDS counter only, never on the
 // next tile's prefetch loads).
-template <int PREC, int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF>
+template <int PREC, int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF, bool TW = false>
 __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Origin &o, int s, const Ops<PREC> &op,
                                        const Prefetch<PREC, R, KIND, INF> &pf, const float2 *stw,
                                        const float2 *twl)
@@ -342,12 +336,12 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
                 typedef typename Elem<OUTF>::raw raw;
                 raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
                 float2 w[4];
-                tw4<R16>(a, twl, (unsigned)(a.tw_level ? o.l1 : o.l0b + t), col + R16 * 4 * g, w);
+                if constexpr (TW) tw4<R16>(a, twl, (unsigned)(a.tw_level ? o.l1 : o.l0b + t), col + R16 * 4 * g, w);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int pt = col + R16 * (4 * g + i);
                     float2 y = make_float2(accr[u][i], acci[u][i]);
-                    if (a.tw_N) {
+                    if constexpr (TW) {
                         y = cmulf(y, w[i]);
                     } else {
                         y.x *= a.last_mul;
@@ -404,7 +398,7 @@ __device__ __forceinline__ void dft16v(float2 *v, int sign)
 }
 
 // One thread = one 16-point column per stage (tile of 4096 points / 256 threads).
-template <int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF>
+template <int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF, bool TW = false>
 __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const Origin &o, int s,
                                          const Prefetch<PREC_FP32, R, KIND, INF> &pf, const float2 *stw,
                                          const float2 *twl)
@@ -437,13 +431,13 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
             float2 w[4];
-            tw4<4 * R16>(a, twl, mult, col + R16 * q4, w);
+            if constexpr (TW) tw4<4 * R16>(a, twl, mult, col + R16 * q4, w);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int tq = q4 + 4 * k;
                 const int pt = col + R16 * tq;
                 float2 y = v[tq];
-                if (a.tw_N) {
+                if constexpr (TW) {
                     y = cmulf(y, w[k]);
                 } else {
                     y.x *= a.last_mul;
@@ -530,7 +524,7 @@ __device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, fl
 // unscaled tables: this stage is VALU, no x16 operand factor), the tail's stage scale and
 // the output conversion. Lanes walk the points of a line (ROWS) or the lines (the
 // transposed and strided kinds), whichever is contiguous in the output.
-template <int R, int KIND, int RADIX, int NW, int OUTF>
+template <int R, int KIND, int RADIX, int NW, int OUTF, bool TW>
 __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, const Origin &o, const float2 *twl)
 {
     typedef Geo<R, KIND, NW> G;
@@ -556,14 +550,14 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
 #pragma unroll
         for (int q4 = 0; q4 < RADIX; q4 += 4) {
             float2 w[4];
-            if constexpr (RADIX >= 4)
+            if constexpr (TW && RADIX >= 4)
                 tw4<COLS, false>(a, twl, mult, col + COLS * q4, w);
 #pragma unroll
             for (int kk = 0; kk < (RADIX < 4 ? RADIX : 4); ++kk) {
                 const int tq = q4 + kk;
                 const int pt = col + COLS * tq;
                 float2 y = make_float2(v[tq].x * a.tail_mul, v[tq].y * a.tail_mul);
-                if (a.tw_N) {
+                if constexpr (TW) {
                     if constexpr (RADIX >= 4) {
                         y = cmulf(y, w[kk]);
                     } else {
@@ -578,40 +572,19 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
 }
 
 // ------------------------------------------------------------------ kernel
-// R = 16^n * TAIL (TAIL in {1,2,4,8}, n >= 2): schedule [16, TAIL?, 16, ..., 16].
-// Persistent: workgroup b transforms tiles b, b + gridDim.x, ...; the first-stage
-// operands of the next tile are loaded before the current tile is transformed.
-template <int PREC, int R, int KIND, int INF, int OUTF>
-__global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(TileArgs a)
+// The persistent tile loop. TW: the last stage multiplies by the four-step twiddles
+// (a.tw_N != 0); the loop is instantiated once per case so neither epilogue carries the
+// other's registers or selects.
+template <int PREC, int R, int KIND, int INF, int OUTF, bool TW>
+__device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const float2 *stw, const float2 *s_tw,
+                                           const Ops<PREC> &op)
 {
     typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
-    __shared__ float2 lds[G::SIZE];
-    __shared__ float2 s_stw[G::STW > 0 ? G::STW : 1];
-    __shared__ float2 s_tw[2 * kFastTw];
     constexpr int L2 = __builtin_ctz(R);
     constexpr int NS = L2 / 4;
     constexpr int TAIL = 1 << (L2 % 4);
-    static_assert(NS >= 2 || (NS == 1 && TAIL > 1), "fast kernels need R >= 32");
     constexpr int MAP_IN = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
     constexpr int MAP_OUT = KIND == KIND_ROWS ? MAP_POINTS : MAP_LINES;
-
-    // tables: stage twiddles and (small) four-step tables into LDS
-    const float2 *stw = a.stage_tw_s;
-    if constexpr (G::STW > 0) {
-        for (int i = threadIdx.x; i < R; i += G::NT) s_stw[i] = a.stage_tw_s[i];
-        stw = s_stw;
-    }
-    if (a.tw_N && a.tw_lds) {
-        const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
-        // the tail-as-last stage (NS == 1) is VALU: it takes the unscaled lo table
-        const float2 *lo = NS == 1 ? a.tw_lo : a.tw_lo_s;
-        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = lo[i];
-        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[kFastTw + i] = a.tw_hi[i];
-    }
-    Ops<PREC> op;
-    if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
-    __syncthreads();
-
     // Each of the 8 XCDs (workgroups are dealt to them round-robin) walks a contiguous
     // range of tiles, so tiles that share cache lines meet in the same L2.
     const long long ntiles = (long long)((a.c0 + G::T - 1) / G::T) * a.c1 * a.c2;
@@ -645,7 +618,7 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
         s = 16;
         if constexpr (NS == 1) {
             // R = 16 * TAIL: the tail is the last stage and stores to global memory
-            tstage_last<R, KIND, TAIL, G::NW, OUTF>(lds, a, o, s_tw);
+            tstage_last<R, KIND, TAIL, G::NW, OUTF, TW>(lds, a, o, s_tw);
         } else {
             if constexpr (TAIL > 1) {
                 tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul);
@@ -660,15 +633,53 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
                 s *= 16;
             }
             if constexpr (PREC == PREC_FP32)
-                vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, cur, stw, s_tw);
+                vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF, TW>(lds, a, o, s, cur, stw, s_tw);
             else
-                mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF>(lds, a, o, s, op, cur, stw, s_tw);
+                mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF, TW>(lds, a, o, s, op, cur, stw, s_tw);
         }
         __syncthreads();  // the next tile's first stage overwrites LDS
         tile = next;
         o = on;
         cur = nxt;
     }
+}
+
+// R = 16^n * TAIL (TAIL in {1,2,4,8}, n >= 2): schedule [16, TAIL?, 16, ..., 16].
+// Persistent: workgroup b transforms tiles b, b + gridDim.x, ...; the first-stage
+// operands of the next tile are loaded before the current tile is transformed.
+template <int PREC, int R, int KIND, int INF, int OUTF>
+__global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(TileArgs a)
+{
+    typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
+    __shared__ float2 lds[G::SIZE];
+    __shared__ float2 s_stw[G::STW > 0 ? G::STW : 1];
+    __shared__ float2 s_tw[2 * kFastTw];
+    constexpr int L2 = __builtin_ctz(R);
+    constexpr int NS = L2 / 4;
+    constexpr int TAIL = 1 << (L2 % 4);
+    static_assert(NS >= 2 || (NS == 1 && TAIL > 1), "fast kernels need R >= 32");
+
+    // tables: stage twiddles and (small) four-step tables into LDS
+    const float2 *stw = a.stage_tw_s;
+    if constexpr (G::STW > 0) {
+        for (int i = threadIdx.x; i < R; i += G::NT) s_stw[i] = a.stage_tw_s[i];
+        stw = s_stw;
+    }
+    if (a.tw_N && a.tw_lds) {
+        const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
+        // the tail-as-last stage (NS == 1) is VALU: it takes the unscaled lo table
+        const float2 *lo = NS == 1 ? a.tw_lo : a.tw_lo_s;
+        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = lo[i];
+        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[kFastTw + i] = a.tw_hi[i];
+    }
+    Ops<PREC> op;
+    if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
+    __syncthreads();
+
+    if (a.tw_N)
+        fast_tiles<PREC, R, KIND, INF, OUTF, true>(a, lds, stw, s_tw, op);
+    else
+        fast_tiles<PREC, R, KIND, INF, OUTF, false>(a, lds, stw, s_tw, op);
 }
 
 template <int PREC, int KIND, int INF, int OUTF>

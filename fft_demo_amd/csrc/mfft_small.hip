@@ -32,11 +32,9 @@ __device__ __forceinline__ void st(void *p, long long i, float2 v)
     if constexpr (FMT == FMT_C32) {
         reinterpret_cast<float2 *>(p)[i] = v;
     } else if constexpr (FMT == FMT_C16) {
-        reinterpret_cast<uint32_t *>(p)[i] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.x) |
-                                             ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v.y) << 16);
+        reinterpret_cast<uint32_t *>(p)[i] = pack_c16(v.x, v.y);
     } else {
-        reinterpret_cast<uint32_t *>(p)[i] = (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.x) |
-                                             ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v.y) << 16);
+        reinterpret_cast<uint32_t *>(p)[i] = pack_cbf16(v.x, v.y);
     }
 }
 

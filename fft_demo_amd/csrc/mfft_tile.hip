@@ -38,14 +38,7 @@ __device__ __forceinline__ float2 cload1(const void *base, long long i, int fmt)
 
 __device__ __forceinline__ uint32_t pack2(float re, float im, int fmt)
 {
-    if (fmt == FMT_C16) {
-        uint16_t a = __builtin_bit_cast(uint16_t, (_Float16)re);
-        uint16_t b = __builtin_bit_cast(uint16_t, (_Float16)im);
-        return (uint32_t)a | ((uint32_t)b << 16);
-    }
-    uint16_t a = __builtin_bit_cast(uint16_t, (__bf16)re);
-    uint16_t b = __builtin_bit_cast(uint16_t, (__bf16)im);
-    return (uint32_t)a | ((uint32_t)b << 16);
+    return fmt == FMT_C16 ? pack_c16(re, im) : pack_cbf16(re, im);
 }
 
 __device__ __forceinline__ void cstore1(void *base, long long i, int fmt, float2 v)
@@ -147,21 +140,23 @@ __device__ __forceinline__ void dft16_mfma(const float2 x[4], const DftOps *ops,
         ar = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, b, ar, 0, 0, 0);
         ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, b, ai, 0, 0, 0);
     } else {
-        h8 bh;
+        h8 bh, bl;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            bh[2 * k] = (_Float16)x[k].x;
-            bh[2 * k + 1] = (_Float16)x[k].y;
+            pk_h2 hv, lv;
+            if constexpr (PREC == PREC_SPLIT) {
+                split_c16(x[k].x, x[k].y, hv, lv);
+                bl[2 * k] = lv.x;
+                bl[2 * k + 1] = lv.y;
+            } else {
+                hv = __builtin_convertvector((pk_f2){x[k].x, x[k].y}, pk_h2);
+            }
+            bh[2 * k] = hv.x;
+            bh[2 * k + 1] = hv.y;
         }
         h8 Ar = *reinterpret_cast<const h8 *>(ops->hr[lane]);
         h8 Ai = *reinterpret_cast<const h8 *>(ops->hi_[lane]);
         if constexpr (PREC == PREC_SPLIT) {
-            h8 bl;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                bl[2 * k] = (_Float16)(x[k].x - (float)bh[2 * k]);
-                bl[2 * k + 1] = (_Float16)(x[k].y - (float)bh[2 * k + 1]);
-            }
             h8 Lr = *reinterpret_cast<const h8 *>(ops->lr[lane]);
             h8 Li = *reinterpret_cast<const h8 *>(ops->li[lane]);
             ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(Lr, bh, ar, 0, 0, 0);
