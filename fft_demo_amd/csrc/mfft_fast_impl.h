@@ -101,21 +101,69 @@ struct Origin {
     long long in_base, out_base;
 };
 
-__device__ __forceinline__ Origin origin(const TileArgs &a, int T, long long tile)
-{
-    int tiles0 = (a.c0 + T - 1) / T;
-    long long r = tile / tiles0;
-    int t0 = (int)(tile - r * tiles0);
-    int l1 = (int)(r % a.c1);
-    long long l2 = r / a.c1;
-    Origin o;
-    o.l0b = t0 * T;
-    o.l1 = l1;
-    o.nv = min(T, a.c0 - o.l0b);
-    o.in_base = (long long)o.l0b * a.in_s0 + (long long)l1 * a.in_s1 + l2 * a.in_s2;
-    o.out_base = (long long)o.l0b * a.out_s0 + (long long)l1 * a.out_s1 + l2 * a.out_s2;
-    return o;
-}
+// The tiles of one workgroup form the progression tile0, tile0 + step, ... The walker keeps
+// the current tile as (t0, l1, l2) with tile = t0 + tiles0 (l1 + c1 l2), the step split the
+// same way, and the element offsets of the tile's first line; advancing adds the parts with
+// at most one carry each, so the loop does no division (the scalar unit is shared by all
+// waves of the compute unit, and each wave walks the same tiles).
+struct TileWalk {
+    int t0, l1, s0, s1, tiles0;
+    long long l2, s2;
+    long long in_base, out_base;
+    long long in_a, in_b, in_c, out_a, out_b, out_c;
+
+    __device__ __forceinline__ void init(const TileArgs &a, int T, long long tile, long long step)
+    {
+        tiles0 = (a.c0 + T - 1) / T;
+        long long r = tile / tiles0;
+        t0 = (int)(tile - r * tiles0);
+        l1 = (int)(r % a.c1);
+        l2 = r / a.c1;
+        r = step / tiles0;
+        s0 = (int)(step - r * tiles0);
+        s1 = (int)(r % a.c1);
+        s2 = r / a.c1;
+        const long long ti = (long long)T * a.in_s0, to = (long long)T * a.out_s0;
+        in_base = t0 * ti + l1 * a.in_s1 + l2 * a.in_s2;
+        out_base = t0 * to + l1 * a.out_s1 + l2 * a.out_s2;
+        in_a = s0 * ti + s1 * a.in_s1 + s2 * a.in_s2;
+        out_a = s0 * to + s1 * a.out_s1 + s2 * a.out_s2;
+        in_b = a.in_s1 - tiles0 * ti;
+        out_b = a.out_s1 - tiles0 * to;
+        in_c = a.in_s2 - a.c1 * a.in_s1;
+        out_c = a.out_s2 - a.c1 * a.out_s1;
+    }
+    __device__ __forceinline__ Origin origin(const TileArgs &a, int T) const
+    {
+        Origin o;
+        o.l0b = t0 * T;
+        o.l1 = l1;
+        o.nv = min(T, a.c0 - o.l0b);
+        o.in_base = in_base;
+        o.out_base = out_base;
+        return o;
+    }
+    __device__ __forceinline__ void advance(const TileArgs &a)
+    {
+        t0 += s0;
+        l1 += s1;
+        l2 += s2;
+        in_base += in_a;
+        out_base += out_a;
+        if (t0 >= tiles0) {
+            t0 -= tiles0;
+            ++l1;
+            in_base += in_b;
+            out_base += out_b;
+        }
+        if (l1 >= a.c1) {
+            l1 -= a.c1;
+            ++l2;
+            in_base += in_c;
+            out_base += out_c;
+        }
+    }
+};
 
 // ------------------------------------------------------------------ DFT16 on MFMA
 // Per-lane A operands of the DFT16 (loaded once per kernel).
@@ -601,13 +649,16 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const
         end = ntiles;
     }
     Prefetch<PREC, R, KIND, INF> cur, nxt;
-    Origin o = origin(a, G::T, tile < end ? tile : 0);
+    TileWalk walk;
+    walk.init(a, G::T, tile < end ? tile : 0, step);
+    Origin o = walk.origin(a, G::T);
     if (tile < end) cur.load(a, o);
     while (tile < end) {
         const long long next = tile + step;
         Origin on = o;
         if (next < end) {
-            on = origin(a, G::T, next);
+            walk.advance(a);
+            on = walk.origin(a, G::T);
             nxt.load(a, on);
         }
         int s = 1;
