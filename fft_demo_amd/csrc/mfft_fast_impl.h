@@ -240,6 +240,16 @@ __device__ __forceinline__ void dft16(const float2 *x, const h8 &raw16, const b8
     }
 }
 
+// Wait for every outstanding vector-memory operation of this wave. Called in the last stage
+// before the tile's stores: what is then outstanding is the next tile's prefetch (issued a
+// whole tile earlier) and the previous tile's stores. Loads and stores share the counter,
+// so a wait for the prefetched registers placed after this tile's stores would also wait
+// for those stores, and one placed after the following tile's prefetch would wait for that.
+__device__ __forceinline__ void wait_prefetch()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt unconstrained
+}
+
 // ------------------------------------------------------------------ first-stage loads
 // The raw (undecoded) first-stage operands of one tile, held in registers so the next
 // tile's loads can be in flight while the current tile is transformed.
@@ -250,7 +260,18 @@ struct Prefetch {
     static constexpr int N = PREC == PREC_FP32 ? 16 : 4 * G::MAXP;
     raw w[N];
 
+    // fp32: full tiles (every line valid, the common case) load without per-lane masks
+    // (faster there; the matrix-core kernels measured faster with the masked form).
     __device__ __forceinline__ void load(const TileArgs &a, const Origin &o)
+    {
+        if (PREC == PREC_FP32 && o.nv == G::T)
+            load_lines<true>(a, o);
+        else
+            load_lines<false>(a, o);
+    }
+
+    template <bool FULL>
+    __device__ __forceinline__ void load_lines(const TileArgs &a, const Origin &o)
     {
         constexpr int MAP = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
         constexpr int R16 = G::R16;
@@ -258,7 +279,7 @@ struct Prefetch {
         if constexpr (PREC == PREC_FP32) {
             int t, col;
             slot_tc<R, G::T, MAP>(threadIdx.x, t, col);
-            const bool ok = t < o.nv;
+            const bool ok = FULL || t < o.nv;
             const raw *base = src + o.in_base + (long long)t * a.in_s0 + (long long)col * a.in_ps;
             const long long step = (long long)R16 * a.in_ps;
 #pragma unroll
@@ -276,7 +297,7 @@ struct Prefetch {
                 const int P = wave + u * G::NW;
                 int t, col;
                 slot_tc<R, G::T, MAP>(P * 16 + c, t, col);
-                const bool ok = (G::PAIRS % G::NW == 0 || P < G::PAIRS) && t < o.nv;
+                const bool ok = (G::PAIRS % G::NW == 0 || P < G::PAIRS) && (FULL || t < o.nv);
                 const raw *base = src + o.in_base + (long long)t * a.in_s0 + (long long)(col + 4 * g * R16) * a.in_ps;
                 const long long step = (long long)R16 * a.in_ps;
 #pragma unroll
@@ -363,6 +384,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
         dft16<PREC, EXACT>(x, r16, rb, op, accr[u], acci[u]);
     }
     if constexpr (!FIRST && !LAST) __syncthreads();  // all reads of the tile precede the writes
+    if constexpr (LAST) wait_prefetch();
 #pragma unroll
     for (int u = 0; u < G::MAXP; ++u) {
         const int P = wave + u * G::NW;
@@ -472,7 +494,9 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
         for (int tq = 0; tq < 16; ++tq)
             lds[G::phys(t, q + s * (16 * p + tq))] = cmulf(v[tq], stw[p * tq * s]);
         __syncthreads();
-    } else if (t < o.nv) {
+    } else {
+        wait_prefetch();
+        if (t >= o.nv) return;
         typedef typename Elem<OUTF>::raw raw;
         raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
         const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
@@ -582,6 +606,7 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
     constexpr int NT = G::NT;
     constexpr int PER = (ITEMS + NT - 1) / NT;
     typedef typename Elem<OUTF>::raw raw;
+    wait_prefetch();
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int I = threadIdx.x + k * NT;
