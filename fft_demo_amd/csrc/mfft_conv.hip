@@ -77,19 +77,20 @@ __global__ void pointwise_mac_kernel(const float2 *X, const float2 *H, float2 *Y
 }
 
 // out[2k][f][y][x] = scale * Re Y[k][f][y + oy][x + ox], out[2k+1][f][y][x] = scale * Im ...
-// (one thread per pair element: each spectrum value is read once for both images)
+// (one thread per pair element: each spectrum value is read once for both images).
+// Workgroup b covers 256 columns of output row b / xb (xb workgroups per row), so the row
+// is decoded once per workgroup and a thread only adds its column.
 template <typename T>
 __global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
-                                   int ox, float scale, long long total)
+                                   int ox, float scale, int xb)
 {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    int x = (int)(i % ow);
-    long long r = i / ow;
-    int y = (int)(r % oh);
-    r /= oh;
-    int f = (int)(r % F);
-    long long k = r / F;
+    const unsigned row = blockIdx.x / (unsigned)xb;
+    const int x = (int)(blockIdx.x - row * (unsigned)xb) * 256 + threadIdx.x;
+    if (x >= ow) return;
+    const unsigned kf = row / (unsigned)oh;
+    const int y = (int)(row - kf * (unsigned)oh);
+    const long long k = kf / (unsigned)F;
+    const int f = (int)(kf - (unsigned)k * (unsigned)F);
     const float2 v = Y[((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox)];
     const long long plane = (long long)F * oh * ow, off = ((long long)f * oh + y) * ow + x;
     out[2 * k * plane + off] = (T)(v.x * scale);
@@ -266,13 +267,16 @@ hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long
 hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
                      float scale, hipStream_t s)
 {
-    long long total = (long long)((B + 1) / 2) * F * oh * ow;
+    const int xb = (int)nblocks(ow);
+    const long long blocks = (long long)((B + 1) / 2) * F * oh * xb;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     if (f64)
-        hipLaunchKernelGGL(crop_unpack_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out, B, F,
-                           oh, ow, Ph, Pw, oy, ox, scale, total);
+        hipLaunchKernelGGL(crop_unpack_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, Y, (double *)out, B, F,
+                           oh, ow, Ph, Pw, oy, ox, scale, xb);
     else
-        hipLaunchKernelGGL(crop_unpack_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out, B, F, oh,
-                           ow, Ph, Pw, oy, ox, scale, total);
+        hipLaunchKernelGGL(crop_unpack_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, Y, (float *)out, B, F,
+                           oh, ow, Ph, Pw, oy, ox, scale, xb);
     return hipGetLastError();
 }
 

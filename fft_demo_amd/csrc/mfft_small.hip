@@ -137,39 +137,61 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
     const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long nl = (long long)a.c0 * a.c1 * a.c2;
     if (id >= nl) return;
-    const int l0 = (int)(id % a.c0);
-    const long long r = id / a.c0;
-    const int l1 = (int)(r % a.c1);
-    const long long l2 = r / a.c1;
+    // line index -> (l0, l1, l2); 32-bit divisions whenever the line count allows
+    int l0, l1;
+    long long l2;
+    if (nl <= 0xffffffffLL) {
+        const unsigned u = (unsigned)id, c0 = (unsigned)a.c0, c1 = (unsigned)a.c1;
+        const unsigned r = u / c0;
+        l0 = (int)(u - r * c0);
+        const unsigned q = r / c1;
+        l1 = (int)(r - q * c1);
+        l2 = q;
+    } else {
+        const long long r = id / a.c0;
+        l0 = (int)(id - r * a.c0);
+        l1 = (int)(r % a.c1);
+        l2 = r / a.c1;
+    }
     const long long ib = (long long)l0 * a.in_s0 + (long long)l1 * a.in_s1 + l2 * a.in_s2;
     const long long ob = (long long)l0 * a.out_s0 + (long long)l1 * a.out_s1 + l2 * a.out_s2;
     float2 v[R];
     if (a.real_src) {
-        // conv: two real images of a pair, zero-padded (replaces a separate packing pass)
+        // conv: two real images of a pair, zero-padded (replaces a separate packing pass).
+        // A line lies in one padded transform: its first point is decoded once into
+        // (transform, y, x) and the others follow by the point stride split into rows and
+        // columns (dx < padded width, so x carries at most once per step).
+        const long long tr = ib / a.real_p;
+        const long long rem = ib - tr * a.real_p;
+        int y = (int)(rem / a.real_pw), x = (int)(rem - (long long)y * a.real_pw);
+        const int dy = (int)(a.in_ps / a.real_pw), dx = (int)(a.in_ps - (long long)dy * a.real_pw);
+        const long long t = a.batch_base + tr;
+        const long long k = t / a.real_c, c = t - k * a.real_c;
+        const long long hw = (long long)a.real_h * a.real_w;
+        const long long i0 = (2 * k * a.real_c + c) * hw;
+        const bool second = a.real_src2 || 2 * k + 1 < a.real_b;
+        const void *src1 = a.real_src2 ? a.real_src2 : a.real_src;
+        const long long i1 = a.real_src2 ? i0 : ((2 * k + 1) * a.real_c + c) * hw;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            const long long e = ib + (long long)j * a.in_ps;
-            const long long tr = e / a.real_p;
-            const long long rem = e - tr * a.real_p;
-            const int y = (int)(rem / a.real_pw), x = (int)(rem - (long long)y * a.real_pw);
-            const long long t = a.batch_base + tr;
-            const long long k = t / a.real_c, c = t - k * a.real_c;
             float2 val = make_float2(0.f, 0.f);
             if (y < a.real_h && x < a.real_w) {
-                const long long hw = (long long)a.real_h * a.real_w, off = (long long)y * a.real_w + x;
-                const long long i0 = (2 * k * a.real_c + c) * hw + off, i1 = ((2 * k + 1) * a.real_c + c) * hw + off;
-                const bool second = a.real_src2 || 2 * k + 1 < a.real_b;
-                const void *src1 = a.real_src2 ? a.real_src2 : a.real_src;
-                const long long j1 = a.real_src2 ? i0 : i1;
+                const long long off = (long long)y * a.real_w + x;
                 if (a.real_f64) {
-                    val.x = (float)reinterpret_cast<const double *>(a.real_src)[i0];
-                    if (second) val.y = (float)reinterpret_cast<const double *>(src1)[j1];
+                    val.x = (float)reinterpret_cast<const double *>(a.real_src)[i0 + off];
+                    if (second) val.y = (float)reinterpret_cast<const double *>(src1)[i1 + off];
                 } else {
-                    val.x = reinterpret_cast<const float *>(a.real_src)[i0];
-                    if (second) val.y = reinterpret_cast<const float *>(src1)[j1];
+                    val.x = reinterpret_cast<const float *>(a.real_src)[i0 + off];
+                    if (second) val.y = reinterpret_cast<const float *>(src1)[i1 + off];
                 }
             }
             v[j] = val;
+            x += dx;
+            y += dy;
+            if (x >= a.real_pw) {
+                x -= a.real_pw;
+                ++y;
+            }
         }
     } else {
 #pragma unroll
@@ -177,8 +199,14 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
     }
     if (a.in_mul) {
         // conv: the pointwise product with the resident filter spectrum, fused into the load
+        // (the line's points stay in one period, so the index is reduced once)
+        long long m = ib % a.in_mul_period;
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = cm(v[j], a.in_mul[(ib + (long long)j * a.in_ps) % a.in_mul_period]);
+        for (int j = 0; j < R; ++j) {
+            v[j] = cm(v[j], a.in_mul[m]);
+            m += a.in_ps;
+            if (m >= a.in_mul_period) m -= a.in_mul_period;
+        }
     }
     if constexpr (R & 1)
         dft_odd<R>(v, a.stage_tw);
