@@ -454,8 +454,9 @@ def serve_section(seconds, sat_requests=2000, p99_bound_ms=2.0):
 def conv_section(steps, warmup):
     """FFT 'same' convolution: 2048x2048 fp32 images U[0,1], batch 16, one 31x31 filter
     (Gaussian sigma 5 normalised to sum 1, times 1 + U[-0.01, 0.01]), padded to 2176;
-    split-fp16 transforms with the filter spectrum resident. Accuracy on sampled rows of
-    two images against the direct FP64 convolution."""
+    split-fp16 transforms with fp16 image spectra and intermediates and the fp32 filter
+    spectrum resident (comparators: split fp16 with fp32 spectra, plain fp32). Accuracy on
+    sampled rows of two images against the direct FP64 convolution."""
     import numpy as np
     import torch
 
@@ -497,13 +498,14 @@ def conv_section(steps, warmup):
 
     rows = (0, 48)
     res = {}
-    for prec in ("split_fp16", "fp32"):
-        op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision=prec)
+    for name, prec, spec16 in (("split_fp16", "split_fp16", True), ("split_fp16_fp32_spectra", "split_fp16", False),
+                               ("fp32", "fp32", False)):
+        op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision=prec, spec_fp16=spec16)
         op.set_filters(gd)
         ms = timed(lambda: op.forward(x, out))
         l2, mx = accuracy()
-        res[prec] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4), "rel_l2": l2,
-                     "rel_max": mx, "pad": list(op.pad)}
+        res[name] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4), "rel_l2": l2,
+                     "rel_max": mx, "pad": list(op.pad), "spectra": "fp16" if spec16 else "fp32"}
         P = op.pad[0] * op.pad[1]
         op.close()
     # labelled external point: torch.fft real transforms (rocFFT), resident filter spectrum
@@ -524,13 +526,15 @@ def conv_section(steps, warmup):
                                    "note": "torch.fft rfft2/irfft2 (rocFFT) with the filter spectrum precomputed"}
     head = res["split_fp16"]
     ms = head["ms_per_batch"]
-    return {"workload": "FFT 'same' conv2d, 2048x2048 fp32 x 16, 31x31 Gaussian filter, pad 2176, split fp16",
+    return {"workload": "FFT 'same' conv2d, 2048x2048 fp32 x 16, 31x31 Gaussian filter, pad 2176, split fp16, "
+                        "fp16 spectra",
             "images_per_s": head["images_per_s"], "ms_per_batch": ms,
             "nominal_gflops": round(B * 2 * 5.0 * P * math.log2(P) / (ms * 1e-3) / 1e9, 1),
             "pad": head["pad"], "rel_l2": head["rel_l2"], "rel_max": head["rel_max"],
             "accuracy_sample": f"rows {rows[0]}..{rows[1] - 1} of images 0 and {B - 1} vs direct FP64 convolution",
             "io_gbs": round(B * H * W * 4 * 2 / (ms * 1e-3) / 1e9, 1),
-            "comparators": {"fp32": res["fp32"], "rocfft_fp32_external": res["rocfft_fp32_external"]}}
+            "comparators": {"split_fp16_fp32_spectra": res["split_fp16_fp32_spectra"], "fp32": res["fp32"],
+                            "rocfft_fp32_external": res["rocfft_fp32_external"]}}
 
 
 def main():

@@ -139,7 +139,7 @@ class ConvDesc(ctypes.Structure):
     _fields_ = [("batch", ctypes.c_int), ("channels", ctypes.c_int), ("filters", ctypes.c_int), ("h", ctypes.c_int),
                 ("w", ctypes.c_int), ("fh", ctypes.c_int), ("fw", ctypes.c_int), ("mode", ctypes.c_int),
                 ("pad_h", ctypes.c_int), ("pad_w", ctypes.c_int), ("precision", ctypes.c_int),
-                ("workspace_limit", ctypes.c_int64)]
+                ("spec_fp16", ctypes.c_int), ("workspace_limit", ctypes.c_int64)]
 
 
 class ConvInfo(ctypes.Structure):

@@ -15,7 +15,7 @@ MODES = {"same": L.CONV_SAME, "valid_corr": L.CONV_VALID_CORR}
 
 class Conv2d:
     def __init__(self, batch, channels, filters, h, w, fh, fw, mode="same", pad=None, precision="split_fp16",
-                 workspace_limit=0):
+                 workspace_limit=0, spec_fp16=False):
         lib = L.load()
         d = L.ConvDesc()
         lib.mfft_conv2d_desc_default(ctypes.byref(d))
@@ -25,6 +25,7 @@ class Conv2d:
             d.pad_h, d.pad_w = (pad, pad) if isinstance(pad, int) else pad
         d.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         d.workspace_limit = workspace_limit
+        d.spec_fp16 = 1 if spec_fp16 else 0  # fp16 image spectra and intermediates (forward only)
         h_ = ctypes.c_void_p()
         L.check(lib.mfft_conv2d_create(ctypes.byref(h_), ctypes.byref(d)), "mfft_conv2d_create")
         self._h = h_

@@ -8,16 +8,37 @@
 // (quant_p1/p2/p3 :1058-1098) -- not needed here: spectra stay fp32 and the forward
 // transform is scaled per stage -- and the crop/scale readout
 // (read_out_scale_double2double_kernel :1390-1397).
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include "mfft_common.h"
 
 namespace mfft {
 
+// Image and product spectra are complex fp32 (FMT_C32) or fp16 (FMT_C16) per operator.
+template <int FMT>
+__device__ __forceinline__ float2 spec_ld(const void *p, long long i)
+{
+    if constexpr (FMT == FMT_C32) {
+        return reinterpret_cast<const float2 *>(p)[i];
+    } else {
+        const uint32_t w = reinterpret_cast<const uint32_t *>(p)[i];
+        return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)),
+                           (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
+    }
+}
+
+template <int FMT>
+__device__ __forceinline__ void spec_st(void *p, long long i, float2 v)
+{
+    if constexpr (FMT == FMT_C32)
+        reinterpret_cast<float2 *>(p)[i] = v;
+    else
+        reinterpret_cast<uint32_t *>(p)[i] = pack_c16(v.x, v.y);
+}
+
 // Two real images (batch items 2k and 2k+1, same channel) become the real and imaginary
 // parts of one complex image, zero-padded to Ph x Pw at the origin.
-// src: real [B][C][h][w] (fp32 or fp64); dst: complex fp32 [pairs][C][Ph][Pw].
-template <typename T>
-__global__ void pack_pairs_kernel(const T *src, float2 *dst, int B, int C, int h, int w, int Ph, int Pw, long long total)
+// src: real [B][C][h][w] (fp32 or fp64); dst: complex [pairs][C][Ph][Pw] (FMT).
+template <typename T, int FMT>
+__global__ void pack_pairs_kernel(const T *src, void *dst, int B, int C, int h, int w, int Ph, int Pw, long long total)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -34,7 +55,7 @@ __global__ void pack_pairs_kernel(const T *src, float2 *dst, int B, int C, int h
         v.x = (float)src[b0 * C * h * w + off];
         if (b1 < B) v.y = (float)src[b1 * C * h * w + off];
     }
-    dst[i] = v;
+    spec_st<FMT>(dst, i, v);
 }
 
 // Filter [F][C][fh][fw] real -> complex fp32 [F][C][Ph][Pw], optionally rotated by 180
@@ -56,8 +77,9 @@ __global__ void pack_filters_kernel(const T *src, float2 *dst, int fh, int fw, i
     dst[i] = v;
 }
 
-// Y[k][f][p] = sum_c X[k][c][p] * H[f][c][p]
-__global__ void pointwise_mac_kernel(const float2 *X, const float2 *H, float2 *Y, int C, int F, long long P,
+// Y[k][f][p] = sum_c X[k][c][p] * H[f][c][p]   (X, Y: FMT; H: fp32)
+template <int FMT>
+__global__ void pointwise_mac_kernel(const void *X, const float2 *H, void *Y, int C, int F, long long P,
                                      long long total)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,20 +90,20 @@ __global__ void pointwise_mac_kernel(const float2 *X, const float2 *H, float2 *Y
     long long k = r / F;
     float re = 0.f, im = 0.f;
     for (int c = 0; c < C; ++c) {
-        float2 a = X[(k * C + c) * P + p];
+        float2 a = spec_ld<FMT>(X, (k * C + c) * P + p);
         float2 b = H[((long long)f * C + c) * P + p];
         re += a.x * b.x - a.y * b.y;
         im += a.x * b.y + a.y * b.x;
     }
-    Y[i] = make_float2(re, im);
+    spec_st<FMT>(Y, i, make_float2(re, im));
 }
 
 // out[2k][f][y][x] = scale * Re Y[k][f][y + oy][x + ox], out[2k+1][f][y][x] = scale * Im ...
 // (one thread per pair element: each spectrum value is read once for both images).
 // Workgroup b covers 256 columns of output row b / xb (xb workgroups per row), so the row
 // is decoded once per workgroup and a thread only adds its column.
-template <typename T>
-__global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
+template <typename T, int FMT>
+__global__ void crop_unpack_kernel(const void *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
                                    int ox, float scale, int xb)
 {
     const unsigned row = blockIdx.x / (unsigned)xb;
@@ -91,15 +113,15 @@ __global__ void crop_unpack_kernel(const float2 *Y, T *out, int B, int F, int oh
     const int y = (int)(row - kf * (unsigned)oh);
     const long long k = kf / (unsigned)F;
     const int f = (int)(kf - (unsigned)k * (unsigned)F);
-    const float2 v = Y[((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox)];
+    const float2 v = spec_ld<FMT>(Y, ((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox));
     const long long plane = (long long)F * oh * ow, off = ((long long)f * oh + y) * ow + x;
     out[2 * k * plane + off] = (T)(v.x * scale);
     if (2 * k + 1 < B) out[(2 * k + 1) * plane + off] = (T)(v.y * scale);
 }
 
 // Two real images a, b (separate buffers, [h][w]) -> one complex image a + i b, zero-padded.
-template <typename T>
-__global__ void pack_two_kernel(const T *a, const T *b, float2 *dst, int h, int w, int Ph, int Pw, long long total)
+template <typename T, int FMT>
+__global__ void pack_two_kernel(const T *a, const T *b, void *dst, int h, int w, int Ph, int Pw, long long total)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -109,36 +131,38 @@ __global__ void pack_two_kernel(const T *a, const T *b, float2 *dst, int h, int 
         long long off = (long long)y * w + x;
         v = make_float2((float)a[off], (float)b[off]);
     }
-    dst[i] = v;
+    spec_st<FMT>(dst, i, v);
 }
 
 // Spectrum X of a + i b (a, b real): A = (X(k) + conj X(-k)) / 2 and B = (X(k) - conj X(-k))
 // / 2i are the spectra of a and b, so Y = A H0 + i B H1 = ((X + X*)/2) H0 + ((X - X*)/2) H1
 // (X* = conj X(-k)) transforms back to (a conv h0) + i (b conv h1): two convolutions with
 // different filters from one forward and one inverse transform.
-__global__ void pointwise_two_kernel(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw)
+template <int FMT>
+__global__ void pointwise_two_kernel(const void *X, const float2 *H0, const float2 *H1, void *Y, int Ph, int Pw)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)Ph * Pw) return;
     int v = (int)(i % Pw), u = (int)(i / Pw);
     int uq = u ? Ph - u : 0, vq = v ? Pw - v : 0;
-    float2 xp = X[i], xq = X[(long long)uq * Pw + vq];
+    float2 xp = spec_ld<FMT>(X, i), xq = spec_ld<FMT>(X, (long long)uq * Pw + vq);
     xq.y = -xq.y;
     float2 s = make_float2(0.5f * (xp.x + xq.x), 0.5f * (xp.y + xq.y));
     float2 d = make_float2(0.5f * (xp.x - xq.x), 0.5f * (xp.y - xq.y));
     float2 h0 = H0[i], h1 = H1[i];
-    Y[i] = make_float2(s.x * h0.x - s.y * h0.y + d.x * h1.x - d.y * h1.y,
-                       s.x * h0.y + s.y * h0.x + d.x * h1.y + d.y * h1.x);
+    spec_st<FMT>(Y, i,
+                 make_float2(s.x * h0.x - s.y * h0.y + d.x * h1.x - d.y * h1.y,
+                             s.x * h0.y + s.y * h0.x + d.x * h1.y + d.y * h1.x));
 }
 
 // out0 = Re Y, out1 = Im Y over the output window at (oy, ox).
-template <typename T>
-__global__ void crop_two_kernel(const float2 *Y, T *out0, T *out1, int oh, int ow, int Pw, int oy, int ox, long long total)
+template <typename T, int FMT>
+__global__ void crop_two_kernel(const void *Y, T *out0, T *out1, int oh, int ow, int Pw, int oy, int ox, long long total)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     int x = (int)(i % ow), y = (int)(i / ow);
-    float2 v = Y[(long long)(y + oy) * Pw + (x + ox)];
+    float2 v = spec_ld<FMT>(Y, (long long)(y + oy) * Pw + (x + ox));
     out0[i] = (T)v.x;
     out1[i] = (T)v.y;
 }
@@ -230,16 +254,26 @@ __global__ void crop_real_kernel(const float2 *Y, T *out, int oh, int ow, int Ph
 
 static inline unsigned nblocks(long long n) { return (unsigned)((n + 255) / 256); }
 
-hipError_t conv_pack_pairs(const void *src, int f64, float2 *dst, int B, int C, int h, int w, int Ph, int Pw,
+template <int FMT>
+static void launch_pack_pairs(const void *src, int f64, void *dst, int B, int C, int h, int w, int Ph, int Pw,
+                              long long total, hipStream_t s)
+{
+    if (f64)
+        hipLaunchKernelGGL((pack_pairs_kernel<double, FMT>), dim3(nblocks(total)), dim3(256), 0, s,
+                           (const double *)src, dst, B, C, h, w, Ph, Pw, total);
+    else
+        hipLaunchKernelGGL((pack_pairs_kernel<float, FMT>), dim3(nblocks(total)), dim3(256), 0, s, (const float *)src,
+                           dst, B, C, h, w, Ph, Pw, total);
+}
+
+hipError_t conv_pack_pairs(const void *src, int f64, void *dst, int fmt, int B, int C, int h, int w, int Ph, int Pw,
                            hipStream_t s)
 {
     long long pairs = (B + 1) / 2, total = pairs * C * (long long)Ph * Pw;
-    if (f64)
-        hipLaunchKernelGGL(pack_pairs_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, (const double *)src, dst,
-                           B, C, h, w, Ph, Pw, total);
+    if (fmt == FMT_C16)
+        launch_pack_pairs<FMT_C16>(src, f64, dst, B, C, h, w, Ph, Pw, total, s);
     else
-        hipLaunchKernelGGL(pack_pairs_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, (const float *)src, dst,
-                           B, C, h, w, Ph, Pw, total);
+        launch_pack_pairs<FMT_C32>(src, f64, dst, B, C, h, w, Ph, Pw, total, s);
     return hipGetLastError();
 }
 
@@ -256,27 +290,42 @@ hipError_t conv_pack_filters(const void *src, int f64, float2 *dst, int FC, int 
     return hipGetLastError();
 }
 
-hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long pairs, int C, int F, long long P,
+hipError_t conv_pointwise(const void *X, const float2 *H, void *Y, int fmt, long long pairs, int C, int F, long long P,
                           hipStream_t s)
 {
     long long total = pairs * F * P;
-    hipLaunchKernelGGL(pointwise_mac_kernel, dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C, F, P, total);
+    if (fmt == FMT_C16)
+        hipLaunchKernelGGL(pointwise_mac_kernel<FMT_C16>, dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C, F, P,
+                           total);
+    else
+        hipLaunchKernelGGL(pointwise_mac_kernel<FMT_C32>, dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C, F, P,
+                           total);
     return hipGetLastError();
 }
 
-hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
-                     float scale, hipStream_t s)
+template <int FMT>
+static void launch_crop(const void *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
+                        float scale, int xb, unsigned blocks, hipStream_t s)
+{
+    if (f64)
+        hipLaunchKernelGGL((crop_unpack_kernel<double, FMT>), dim3(blocks), dim3(256), 0, s, Y, (double *)out, B, F, oh,
+                           ow, Ph, Pw, oy, ox, scale, xb);
+    else
+        hipLaunchKernelGGL((crop_unpack_kernel<float, FMT>), dim3(blocks), dim3(256), 0, s, Y, (float *)out, B, F, oh,
+                           ow, Ph, Pw, oy, ox, scale, xb);
+}
+
+hipError_t conv_crop(const void *Y, int fmt, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
+                     int ox, float scale, hipStream_t s)
 {
     const int xb = (int)nblocks(ow);
     const long long blocks = (long long)((B + 1) / 2) * F * oh * xb;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (f64)
-        hipLaunchKernelGGL(crop_unpack_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, Y, (double *)out, B, F,
-                           oh, ow, Ph, Pw, oy, ox, scale, xb);
+    if (fmt == FMT_C16)
+        launch_crop<FMT_C16>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, (unsigned)blocks, s);
     else
-        hipLaunchKernelGGL(crop_unpack_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, Y, (float *)out, B, F,
-                           oh, ow, Ph, Pw, oy, ox, scale, xb);
+        launch_crop<FMT_C32>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, (unsigned)blocks, s);
     return hipGetLastError();
 }
 
@@ -322,36 +371,61 @@ hipError_t conv_crop_real(const float2 *Y, void *out, int f64, long long planes,
     return hipGetLastError();
 }
 
-hipError_t conv_pack_two(const void *a, const void *b, int f64, float2 *dst, int h, int w, int Ph, int Pw,
+template <int FMT>
+static void launch_pack_two(const void *a, const void *b, int f64, void *dst, int h, int w, int Ph, int Pw,
+                            long long total, hipStream_t s)
+{
+    if (f64)
+        hipLaunchKernelGGL((pack_two_kernel<double, FMT>), dim3(nblocks(total)), dim3(256), 0, s, (const double *)a,
+                           (const double *)b, dst, h, w, Ph, Pw, total);
+    else
+        hipLaunchKernelGGL((pack_two_kernel<float, FMT>), dim3(nblocks(total)), dim3(256), 0, s, (const float *)a,
+                           (const float *)b, dst, h, w, Ph, Pw, total);
+}
+
+hipError_t conv_pack_two(const void *a, const void *b, int f64, void *dst, int fmt, int h, int w, int Ph, int Pw,
                          hipStream_t s)
 {
     long long total = (long long)Ph * Pw;
-    if (f64)
-        hipLaunchKernelGGL(pack_two_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, (const double *)a,
-                           (const double *)b, dst, h, w, Ph, Pw, total);
+    if (fmt == FMT_C16)
+        launch_pack_two<FMT_C16>(a, b, f64, dst, h, w, Ph, Pw, total, s);
     else
-        hipLaunchKernelGGL(pack_two_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, (const float *)a,
-                           (const float *)b, dst, h, w, Ph, Pw, total);
+        launch_pack_two<FMT_C32>(a, b, f64, dst, h, w, Ph, Pw, total, s);
     return hipGetLastError();
 }
 
-hipError_t conv_pointwise_two(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw,
+hipError_t conv_pointwise_two(const void *X, const float2 *H0, const float2 *H1, void *Y, int fmt, int Ph, int Pw,
                               hipStream_t s)
 {
-    hipLaunchKernelGGL(pointwise_two_kernel, dim3(nblocks((long long)Ph * Pw)), dim3(256), 0, s, X, H0, H1, Y, Ph, Pw);
+    if (fmt == FMT_C16)
+        hipLaunchKernelGGL(pointwise_two_kernel<FMT_C16>, dim3(nblocks((long long)Ph * Pw)), dim3(256), 0, s, X, H0, H1,
+                           Y, Ph, Pw);
+    else
+        hipLaunchKernelGGL(pointwise_two_kernel<FMT_C32>, dim3(nblocks((long long)Ph * Pw)), dim3(256), 0, s, X, H0, H1,
+                           Y, Ph, Pw);
     return hipGetLastError();
 }
 
-hipError_t conv_crop_two(const float2 *Y, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy, int ox,
-                         hipStream_t s)
+template <int FMT>
+static void launch_crop_two(const void *Y, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy, int ox,
+                            long long total, hipStream_t s)
 {
-    long long total = (long long)oh * ow;
     if (f64)
-        hipLaunchKernelGGL(crop_two_kernel<double>, dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out0,
+        hipLaunchKernelGGL((crop_two_kernel<double, FMT>), dim3(nblocks(total)), dim3(256), 0, s, Y, (double *)out0,
                            (double *)out1, oh, ow, Pw, oy, ox, total);
     else
-        hipLaunchKernelGGL(crop_two_kernel<float>, dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out0,
+        hipLaunchKernelGGL((crop_two_kernel<float, FMT>), dim3(nblocks(total)), dim3(256), 0, s, Y, (float *)out0,
                            (float *)out1, oh, ow, Pw, oy, ox, total);
+}
+
+hipError_t conv_crop_two(const void *Y, int fmt, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy,
+                         int ox, hipStream_t s)
+{
+    long long total = (long long)oh * ow;
+    if (fmt == FMT_C16)
+        launch_crop_two<FMT_C16>(Y, out0, out1, f64, oh, ow, Pw, oy, ox, total, s);
+    else
+        launch_crop_two<FMT_C32>(Y, out0, out1, f64, oh, ow, Pw, oy, ox, total, s);
     return hipGetLastError();
 }
 

@@ -21,14 +21,14 @@
 #include <string>
 
 namespace mfft {
-hipError_t conv_pack_pairs(const void *src, int f64, float2 *dst, int B, int C, int h, int w, int Ph, int Pw,
+hipError_t conv_pack_pairs(const void *src, int f64, void *dst, int fmt, int B, int C, int h, int w, int Ph, int Pw,
                            hipStream_t s);
 hipError_t conv_pack_filters(const void *src, int f64, float2 *dst, int FC, int fh, int fw, int Ph, int Pw, int rotate,
                              hipStream_t s);
-hipError_t conv_pointwise(const float2 *X, const float2 *H, float2 *Y, long long pairs, int C, int F, long long P,
+hipError_t conv_pointwise(const void *X, const float2 *H, void *Y, int fmt, long long pairs, int C, int F, long long P,
                           hipStream_t s);
-hipError_t conv_crop(const float2 *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
-                     float scale, hipStream_t s);
+hipError_t conv_crop(const void *Y, int fmt, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
+                     int ox, float scale, hipStream_t s);
 hipError_t conv_pointwise_gw(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph, int Pw,
                              hipStream_t s);
 hipError_t conv_pointwise_gi(const float2 *D, const float2 *H, float2 *R, long long pairs, int C, int F, long long P,
@@ -37,18 +37,19 @@ hipError_t conv_crop_cyclic(const float2 *Y, void *out, int f64, int B, int C, i
                             int ox, float scale, hipStream_t s);
 hipError_t conv_crop_real(const float2 *Y, void *out, int f64, long long planes, int oh, int ow, int Ph, int Pw,
                           float scale, hipStream_t s);
-hipError_t conv_pack_two(const void *a, const void *b, int f64, float2 *dst, int h, int w, int Ph, int Pw,
+hipError_t conv_pack_two(const void *a, const void *b, int f64, void *dst, int fmt, int h, int w, int Ph, int Pw,
                          hipStream_t s);
-hipError_t conv_pointwise_two(const float2 *X, const float2 *H0, const float2 *H1, float2 *Y, int Ph, int Pw,
+hipError_t conv_pointwise_two(const void *X, const float2 *H0, const float2 *H1, void *Y, int fmt, int Ph, int Pw,
                               hipStream_t s);
-hipError_t conv_crop_two(const float2 *Y, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy, int ox,
-                         hipStream_t s);
+hipError_t conv_crop_two(const void *Y, int fmt, void *out0, void *out1, int f64, int oh, int ow, int Pw, int oy,
+                         int ox, hipStream_t s);
 }  // namespace mfft
 
 struct mfft_conv2d_s {
     mfft_conv2d_desc d{};
     int Ph = 0, Pw = 0, oh = 0, ow = 0, oy = 0, ox = 0;
     long long pairs = 0;
+    int spec_fmt = MFFT_FMT_C32;  // image and product spectra X, Y (the filter spectra H stay fp32)
     mfft_plan fwd_img = nullptr, fwd_filt = nullptr, inv = nullptr;
     float2 *H = nullptr, *X = nullptr, *Y = nullptr;
     // backward (built on its first call): delta transform, inverses over (filter, channel)
@@ -62,8 +63,7 @@ struct mfft_conv2d_s {
 
 namespace {
 
-thread_local std::string c_err;
-
+// Errors of the conv entry points go to the thread's mfft_last_error() message.
 int cfail(int code, const char *fmt, ...)
 {
     char buf[256];
@@ -71,7 +71,7 @@ int cfail(int code, const char *fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    c_err = buf;
+    mfft::set_last_error(buf);
     return code;
 }
 
@@ -142,8 +142,10 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     if (d->mode != MFFT_CONV_SAME && d->mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_INVALID, "mode");
     if (d->mode == MFFT_CONV_VALID_CORR && (d->fh > d->h || d->fw > d->w))
         return cfail(MFFT_ERR_INVALID, "filter larger than the image in 'valid' mode");
+    if (d->spec_fp16 != 0 && d->spec_fp16 != 1) return cfail(MFFT_ERR_INVALID, "spec_fp16");
     mfft_conv2d op = new mfft_conv2d_s;
     op->d = *d;
+    op->spec_fmt = d->spec_fp16 ? MFFT_FMT_C16 : MFFT_FMT_C32;
     int need_h = d->h + d->fh - 1, need_w = d->w + d->fw - 1;
     if (d->mode == MFFT_CONV_VALID_CORR) {
         // circular wrap-around only touches outputs outside the 'valid' window
@@ -171,17 +173,22 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     mfft_desc fd;
     mfft_desc_default(&fd);
     fd.precision = d->precision;
-    fd.in_fmt = fd.out_fmt = MFFT_FMT_C32;
+    fd.in_fmt = fd.out_fmt = op->spec_fmt;
+    if (op->spec_fmt == MFFT_FMT_C16) fd.mid_fmt = MFFT_FMT_C16;
     fd.workspace_limit = d->workspace_limit;
     fd.scale = MFFT_SCALE_STAGE;  // image spectra carry 1/(Ph*Pw): the inverse is then unit
     int rc = mfft_plan_create_2d(&op->fwd_img, op->Ph, op->Pw, op->pairs * d->channels, &fd);
     if (!rc) {
+        fd.direction = MFFT_INVERSE;
         fd.scale = MFFT_SCALE_NONE;
-        rc = mfft_plan_create_2d(&op->fwd_filt, op->Ph, op->Pw, (int64_t)d->filters * d->channels, &fd);
+        rc = mfft_plan_create_2d(&op->inv, op->Ph, op->Pw, op->pairs * d->filters, &fd);
     }
     if (!rc) {
-        fd.direction = MFFT_INVERSE;
-        rc = mfft_plan_create_2d(&op->inv, op->Ph, op->Pw, op->pairs * d->filters, &fd);
+        // the resident filter spectra stay fp32 whatever the image spectra
+        fd.direction = MFFT_FORWARD;
+        fd.in_fmt = fd.out_fmt = MFFT_FMT_C32;
+        fd.mid_fmt = -1;
+        rc = mfft_plan_create_2d(&op->fwd_filt, op->Ph, op->Pw, (int64_t)d->filters * d->channels, &fd);
     }
     if (rc) {
         std::string m = mfft_last_error();
@@ -190,7 +197,8 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     }
     long long P = (long long)op->Ph * op->Pw;
     op->spec_bytes = (long long)d->filters * d->channels * P * 8;
-    long long xb = op->pairs * d->channels * P * 8, yb = op->pairs * d->filters * P * 8;
+    const long long eb = op->spec_fmt == MFFT_FMT_C16 ? 4 : 8;
+    long long xb = op->pairs * d->channels * P * eb, yb = op->pairs * d->filters * P * eb;
     if (hipMalloc(&op->H, op->spec_bytes) != hipSuccess || hipMalloc(&op->X, xb) != hipSuccess ||
         hipMalloc(&op->Y, yb) != hipSuccess) {
         mfft_conv2d_destroy(op);
@@ -239,8 +247,8 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
         f.real_w = d.w;
         rc = mfft::exec_fused(op->fwd_img, op->X, op->X, op->pairs * d.channels, s, f);
     } else {
-        if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw,
-                                  s) != hipSuccess)
+        if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, op->spec_fmt, d.batch, d.channels, d.h, d.w,
+                                  op->Ph, op->Pw, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack");
         rc = mfft_exec(op->fwd_img, op->X, op->X, s);
     }
@@ -255,13 +263,13 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
         f.in_mul_period = P;
         rc = mfft::exec_fused(op->inv, op->X, op->Y, op->pairs, s, f);
     } else {
-        if (mfft::conv_pointwise(op->X, H, op->Y, op->pairs, d.channels, nf, P, s) != hipSuccess)
+        if (mfft::conv_pointwise(op->X, H, op->Y, op->spec_fmt, op->pairs, d.channels, nf, P, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pointwise");
         rc = mfft_exec_ex(op->inv, op->Y, op->Y, op->pairs * nf, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    if (mfft::conv_crop(op->Y, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph, op->Pw, op->oy,
-                        op->ox, 1.0f, s) != hipSuccess)
+    if (mfft::conv_crop(op->Y, op->spec_fmt, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
+                        op->Pw, op->oy, op->ox, 1.0f, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
 }
@@ -302,18 +310,19 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
         f.real_w = d.w;
         rc = mfft::exec_fused(op->fwd_img, op->X, op->X, 1, s, f);
     } else {
-        if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, d.h, d.w, op->Ph, op->Pw, s) != hipSuccess)
+        if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, op->spec_fmt, d.h, d.w, op->Ph, op->Pw, s) !=
+            hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack");
         rc = mfft_exec_ex(op->fwd_img, op->X, op->X, 1, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    if (mfft::conv_pointwise_two(op->X, op->H + filter0 * P, op->H + filter1 * P, op->Y, op->Ph, op->Pw, s) !=
-        hipSuccess)
+    if (mfft::conv_pointwise_two(op->X, op->H + filter0 * P, op->H + filter1 * P, op->Y, op->spec_fmt, op->Ph, op->Pw,
+                                 s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "pointwise");
     rc = mfft_exec_ex(op->inv, op->Y, op->Y, 1, nullptr, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    if (mfft::conv_crop_two(op->Y, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy, op->ox, s) !=
-        hipSuccess)
+    if (mfft::conv_crop_two(op->Y, op->spec_fmt, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy,
+                            op->ox, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
 }
@@ -357,6 +366,7 @@ int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int 
 {
     if (!op || !delta || (!grad_in && !grad_w) || (grad_w && !in)) return cfail(MFFT_ERR_INVALID, "null argument");
     if (op->d.mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_UNSUPPORTED, "backward of the 'valid' correlation mode only");
+    if (op->spec_fmt != MFFT_FMT_C32) return cfail(MFFT_ERR_UNSUPPORTED, "backward needs fp32 spectra (spec_fp16 = 0)");
     if (grad_in && !op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
     std::lock_guard<std::mutex> lk(op->mu);
     int rc = backward_setup(op);
@@ -366,12 +376,14 @@ int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int 
     const int f64 = real_fmt == MFFT_REAL_F64;
     const long long P = (long long)op->Ph * op->Pw;
     // delta spectra (scaled 1/P) in Y
-    if (mfft::conv_pack_pairs(delta, f64, op->Y, d.batch, d.filters, op->oh, op->ow, op->Ph, op->Pw, s) != hipSuccess)
+    if (mfft::conv_pack_pairs(delta, f64, op->Y, MFFT_FMT_C32, d.batch, d.filters, op->oh, op->ow, op->Ph, op->Pw, s) !=
+        hipSuccess)
         return cfail(MFFT_ERR_HIP, "pack delta");
     rc = mfft_exec(op->fwd_delta, op->Y, op->Y, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
     if (grad_w) {
-        if (mfft::conv_pack_pairs(in, f64, op->X, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw, s) != hipSuccess)
+        if (mfft::conv_pack_pairs(in, f64, op->X, MFFT_FMT_C32, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw, s) !=
+            hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack input");
         rc = mfft_exec(op->fwd_img, op->X, op->X, s);
         if (rc) return cfail(rc, "%s", mfft_last_error());

@@ -23,4 +23,7 @@ bool plan_first_pass_fusable(mfft_plan p);
 // the first pass is not fusable.
 int exec_fused(mfft_plan p, const void *in, void *out, int64_t batch, void *stream, const ExecFuse &f);
 
+// Set the calling thread's message returned by mfft_last_error().
+void set_last_error(const char *msg);
+
 }  // namespace mfft

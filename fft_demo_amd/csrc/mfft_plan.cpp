@@ -1008,6 +1008,8 @@ const char *mfft_version(void) { return "mfft 0.1 (gfx950 matrix-core FFT)"; }
 
 namespace mfft {
 
+void set_last_error(const char *msg) { g_err = msg ? msg : ""; }
+
 bool plan_first_pass_fusable(mfft_plan p)
 {
     return p && !p->passes.empty() && p->passes.front().small && p->passes.front().src == BUF_IN;

@@ -207,6 +207,8 @@ ConvSlot *get_conv(mfft_server_s *S, int h, int w, int filter_id, int *index)
         cd.fw = Fw;
         cd.mode = MFFT_CONV_SAME;
         cd.precision = S->prec;
+        // fp16 intermediates asked for the FFT plans: the conv operators keep fp16 spectra too
+        cd.spec_fp16 = !S->d.plain && S->d.mid_fmt == MFFT_FMT_C16 ? 1 : 0;
         cd.workspace_limit = S->slab_bytes;
         {
             mfft_request probe{};

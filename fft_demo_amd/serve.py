@@ -41,8 +41,8 @@ class Server:
 
     precision: "split" (split fp16, default), "fp16", "bf16" or "fp32"; plain=True gives
     the baseline (fp32, one request at a time, each finished before the next starts).
-    mid_fmt="c16" keeps the split plans' intermediates in fp16 (one more rounding per pass
-    boundary, half the intermediate traffic).
+    mid_fmt="c16" keeps the split plans' intermediates, and the conv operators' image
+    spectra, in fp16 (one more rounding per pass boundary, half the intermediate traffic).
     FFT requests carry interleaved complex data in the server's format (fp16 pairs for
     split/fp16, bf16 pairs for bf16, fp32 pairs for fp32 and plain) and produce
     out_scale * DFT(in); conv2d requests carry fp32 real images.

@@ -197,6 +197,10 @@ typedef struct mfft_conv2d_desc {
     int mode;           /* MFFT_CONV_* */
     int pad_h, pad_w;   /* transform size; 0 = smallest supported size >= h + fh - 1 */
     int precision;      /* MFFT_PREC_* of the transforms (default split fp16) */
+    int spec_fp16;      /* 0 (default): image spectra and the transforms' intermediates in fp32;
+                           1: in fp16 (half the bytes of every pass; each pass boundary rounds
+                           to fp16, rel-L2 of the outputs about 1e-4..5e-4). Forward calls
+                           only: mfft_conv2d_backward needs fp32 spectra. */
     int64_t workspace_limit;
 } mfft_conv2d_desc;
 
@@ -292,7 +296,8 @@ typedef struct mfft_server_desc {
     int coalesce_max;      /* small 1D requests per work-queue launch */
     int64_t slab_bytes;    /* workspace per stream (default 512 MiB) */
     int plain;
-    int mid_fmt;           /* intermediates of the FFT plans (mfft_desc.mid_fmt; -1 = default) */
+    int mid_fmt;           /* intermediates of the FFT plans (mfft_desc.mid_fmt; -1 = default);
+                              MFFT_FMT_C16 also gives the conv operators fp16 spectra */
 } mfft_server_desc;
 
 typedef struct mfft_request {

@@ -48,16 +48,46 @@ def test_version_string():
 
 
 def test_struct_layouts_match_headers(tmp_path):
+    """Every ctypes mirror has the C struct's size and each field's offset (a field added
+    into padding keeps the size, so offsets are checked too)."""
     from fft_demo_amd import _lib
 
+    pairs = [("mfft_desc", _lib.Desc), ("mfft_info", _lib.Info), ("mfft_pass_info", _lib.PassInfo),
+             ("mfft_conv2d_desc", _lib.ConvDesc), ("mfft_conv2d_info", _lib.ConvInfo),
+             ("mfft_batch_item", _lib.BatchItem), ("mfft_tune_entry", _lib.TuneEntry),
+             ("mfft_server_desc", _lib.ServerDesc), ("mfft_request", _lib.Request),
+             ("mfft_completion", _lib.Completion), ("mfft_server_stats", _lib.ServerStats),
+             ("tcfftHandle", _lib.TcfftHandle), ("tcfftHandle_2d", _lib.TcfftHandle2d)]
+    lines = []
+    want = []
+    for cname, py in pairs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(ctypes.sizeof(py))
+        for f in py._fields_:
+            cfield = {"passes": "pass", "in_": "in"}.get(f[0], f[0])  # Python keywords renamed
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {cfield}));')
+            want.append(getattr(py, f[0]).offset)
+    # every C member has a ctypes field (a missing one can hide in padding)
+    text = "".join(open(os.path.join(INC, h)).read() for h in ("mfft.h", "tcfft_half.h", "tcfft_half_2d.h"))
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    for cname, py in pairs:
+        body = re.search(r"typedef struct " + cname + r"\s*\{(.*?)\}\s*" + cname + ";", text, re.S).group(1)
+        members = set()
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            for part in decl.split(","):
+                members.add(re.findall(r"[A-Za-z_]\w*", re.sub(r"\[.*?\]", "", part))[-1])
+        have = {{"passes": "pass", "in_": "in"}.get(f[0], f[0]) for f in py._fields_}
+        assert members == have, (cname, members ^ have)
     src = tmp_path / "sz.c"
     src.write_text('#include "mfft.h"\n#include "tcfft_half.h"\n#include "tcfft_half_2d.h"\n#include <stdio.h>\n'
-                   'int main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(mfft_desc), sizeof(mfft_info), '
-                   'sizeof(mfft_pass_info), sizeof(tcfftHandle), sizeof(tcfftHandle_2d));}\n')
+                   '#include <stddef.h>\nint main(){' + "".join(lines) + "}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", INC, str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
-    want = [ctypes.sizeof(t) for t in (_lib.Desc, _lib.Info, _lib.PassInfo, _lib.TcfftHandle, _lib.TcfftHandle2d)]
     assert got == want
 
 

@@ -175,3 +175,53 @@ def test_conv_backward_adjoint_identities_large(dev):
     print(f"<y,d>={lhs:.6e} <x,gi>={r_in:.6e} <w,gw>={r_w:.6e} scale={scale:.3e}")
     assert abs(r_in - lhs) <= 1e-4 * scale
     assert abs(r_w - lhs) <= 1e-4 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,C,F,h,w,fh,fw", [(3, 1, 1, 120, 136, 9, 7), (2, 2, 3, 100, 90, 5, 5)])
+def test_conv_same_fp16_spectra(dev, B, C, F, h, w, fh, fw):
+    """fp16 image spectra and intermediates (spec_fp16): one filter and channel (the image
+    load and the filter product fused into the transforms' first passes) and several
+    (separate channel-sum pass). Each pass boundary rounds to fp16, so the tolerance is
+    rel-L2 <= 1e-3 and rel-max <= 1e-2 of each output map against the FP64 oracle."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    img = oracle.uniform(B * C * h * w, 71, 0.0, 1.0).reshape(B, C, h, w)
+    filt = oracle.uniform(F * C * fh * fw, 72, -1.0, 1.0).reshape(F, C, fh, fw)
+    op = Conv2d(B, C, F, h, w, fh, fw, mode="same", spec_fp16=True)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=torch.float32).contiguous())
+    out = op.forward(torch.from_numpy(img).to(device=dev, dtype=torch.float32).contiguous())
+    torch.cuda.synchronize()
+    got = out.double().cpu().numpy()
+    for b in range(B):
+        for f in range(F):
+            ref = sum(oracle.conv2d_same(img[b, c], filt[f, c]) for c in range(C))
+            l2, mx = oracle.err_l2max(got[b, f], ref)
+            print(f"spec_fp16 pad={op.pad} b={b} f={f} rel_l2={l2:.2e} rel_max={mx:.2e}")
+            assert l2 <= 1e-3 and mx <= 1e-2, (b, f, l2, mx)
+
+
+@pytest.mark.gpu
+def test_conv_forward_two_fp16_spectra_and_backward_refused(dev):
+    """forward_two with fp16 spectra against the oracle (tolerance as for spec_fp16 above);
+    the backward pass of such an operator is refused (it needs fp32 spectra)."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    h, w, k, F = 200, 150, 7, 3
+    filt = oracle.uniform(F * k * k, 81, -1.0, 1.0).reshape(F, 1, k, k)
+    imgs = [oracle.uniform(h * w, 82 + i, 0.0, 1.0).reshape(h, w) for i in range(2)]
+    op = Conv2d(1, 1, F, h, w, k, k, mode="valid_corr", spec_fp16=True)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=torch.float32).contiguous())
+    xs = [torch.from_numpy(im).to(device=dev, dtype=torch.float32).contiguous() for im in imgs]
+    o0, o1 = op.forward_two(xs[0], xs[1], 2, 0)
+    torch.cuda.synchronize()
+    for im, f, o in [(imgs[0], 2, o0), (imgs[1], 0, o1)]:
+        l2, mx = oracle.err_l2max(o.double().cpu().numpy(), oracle.corr2d_valid(im, filt[f, 0]))
+        print(f"forward_two spec_fp16 f={f} rel_l2={l2:.2e} rel_max={mx:.2e}")
+        assert l2 <= 1e-3 and mx <= 1e-2
+    with pytest.raises(RuntimeError, match="fp32 spectra"):
+        op.backward(torch.zeros(1, F, h - k + 1, w - k + 1, device=dev), xs[0].reshape(1, 1, h, w))
