@@ -69,13 +69,13 @@ def time_plan(plan, x, out, steps, warmup, stream):
     """Time `steps` back-to-back execs between two HIP events on the exec stream; when an
     exec is shorter than 0.5 ms (the Python wrapper's launch overhead would then leave gaps
     in the queue) a hipGraph holding the `steps` execs is timed too and the faster is reported.
-    A second run of `steps` execs with an event between every launch gives each pass's
-    average launch time (for the roofline). Returns (total_ms, [per-pass avg ms])."""
+    Each pass is then timed on its own: `steps` back-to-back launches of that pass alone
+    (mfft_exec_pass) between two HIP events on the same stream give its average launch time
+    (for the roofline). Returns (total_ms, [per-pass avg ms])."""
     import torch
 
     from fft_demo_amd.hiptime import EventPool
 
-    nl = plan.info["launches_per_exec"]
     npass = len(plan.info["passes"])
     for _ in range(warmup):
         plan.exec(x, out, stream=stream)
@@ -112,23 +112,22 @@ def time_plan(plan, x, out, steps, warmup, stream):
             TIMING = "hipGraph replay of the steps (captured execs), between two HIP events"
         del g
     ends.close()
-    pool = EventPool(steps * (nl + 1))
-    for i in range(steps):
-        rc = lib.mfft_exec_events(plan._h, x.data_ptr(), out.data_ptr(), stream.cuda_stream, pool.slice(i * (nl + 1)),
-                                  nl + 1)
-        if rc != 0:
-            raise RuntimeError(lib.mfft_last_error().decode())
+    per_pass = []
+    ends = EventPool(2)
+    for k in range(npass):
+        ends.record(0, stream.cuda_stream)
+        for _ in range(steps):
+            rc = lib.mfft_exec_pass(plan._h, k, x.data_ptr(), out.data_ptr(), stream.cuda_stream)
+            if rc != 0:
+                raise RuntimeError(lib.mfft_last_error().decode())
+        ends.record(1, stream.cuda_stream)
+        torch.cuda.synchronize()
+        per_pass.append(ends.elapsed_ms(0, 1) / steps)
+    ends.close()
+    # leave `out` holding a full exec's result
+    plan.exec(x, out, stream=stream)
     torch.cuda.synchronize()
-    per_pass = [0.0] * npass
-    per_pass_n = [0] * npass
-    for i in range(steps):
-        base = i * (nl + 1)
-        for k in range(nl):
-            d = pool.elapsed_ms(base + k, base + k + 1)
-            per_pass[k % npass] += d
-            per_pass_n[k % npass] += 1
-    pool.close()
-    return total, [per_pass[i] / max(1, per_pass_n[i]) for i in range(npass)]
+    return total, per_pass
 
 
 def accuracy(plan, xh, out, sample, direction=-1):

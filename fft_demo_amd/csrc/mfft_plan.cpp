@@ -502,9 +502,11 @@ static void *buf_ptr(int which, const void *in, void *out, void *work)
 }
 
 static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s, hipEvent_t *ev,
-                      int &evi, void *work, const mfft::ExecFuse *fz = nullptr, long long b0 = 0)
+                      int &evi, void *work, const mfft::ExecFuse *fz = nullptr, long long b0 = 0, int only = -1)
 {
+    int k = -1;
     for (auto &ps : p->passes) {
+        if (++k != only && only >= 0) continue;
         TileArgs a;
         memset(&a, 0, sizeof a);
         if (fz && &ps == &p->passes.front()) {
@@ -753,7 +755,7 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
 }
 
 static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEvent_t *ev, long long batch = -1,
-                    void *work = nullptr, const mfft::ExecFuse *fz = nullptr)
+                    void *work = nullptr, const mfft::ExecFuse *fz = nullptr, int only = -1)
 {
     std::unique_lock<std::mutex> lk(p->exec_mu, std::defer_lock);
     if (!work) lk.lock();  // the plan's own workspace is shared by its callers
@@ -765,7 +767,7 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
     for (long long b0 = 0; b0 < batch; b0 += p->chunk) {
         long long nb = std::min(p->chunk, batch - b0);
         int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work,
-                            fz, b0);
+                            fz, b0, only);
         if (rc) return rc;
     }
     if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
@@ -916,6 +918,14 @@ int mfft_batcher_exec(mfft_batcher b, const mfft_batch_item *items, int nitems, 
     HIP_TRY(hipEventRecord(sl.done, s));
     sl.used = true;
     return MFFT_OK;
+}
+
+int mfft_exec_pass(mfft_plan p, int pass, const void *in, void *out, void *stream)
+{
+    if (!p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
+    if (pass < 0 || pass >= (int)p->passes.size())
+        return fail(MFFT_ERR_INVALID, "pass %d of %d", pass, (int)p->passes.size());
+    return exec_all(p, in, out, stream, nullptr, -1, nullptr, nullptr, pass);
 }
 
 int mfft_exec_events(mfft_plan p, const void *in, void *out, void *stream, void **events, int nevents)

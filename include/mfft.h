@@ -133,6 +133,12 @@ int mfft_exec(mfft_plan plan, const void *in, void *out, void *stream);
  * can time every kernel of the exec inside its own timed region. */
 int mfft_exec_events(mfft_plan plan, const void *in, void *out, void *stream, void **events, int nevents);
 
+/* Runs pass `pass` (0 .. npasses-1) of an exec on its own, every chunk, on the buffers it
+ * uses in a full exec (in, out or the plan's workspace): a pass reads what the previous
+ * pass left. For per-kernel timing (back-to-back launches of one pass). No counterpart in
+ * the reference. */
+int mfft_exec_pass(mfft_plan plan, int pass, const void *in, void *out, void *stream);
+
 /* Work-queue batching: many transforms of uneven length (n <= 4096, radices
  * {16, 8, 4, 2, 3, 5, 17}) and batch count in ONE launch of a persistent kernel that takes
  * tiles from a device-side counter. Items are contiguous batches in the batcher's formats. */

@@ -117,3 +117,29 @@ def test_short_line_kernels_all_precisions(dev, n, prec):
     kinds = [p["kernel"] for p in plan.info["passes"]]
     print(f"N={n} b={b} {prec}: rel_l2={l2:.3e} rel_max={mx:.3e} passes={[p['R'] for p in plan.info['passes']]} {kinds}")
     assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,batch,mid", [(65536, 8, "c16"), (1 << 20, 2, None)])
+def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid):
+    """mfft_exec_pass run for every pass in order (the per-kernel timing entry point) leaves
+    exactly the output of one mfft_exec (bit for bit), on a two-pass and a longer plan."""
+    import torch
+
+    import fft_demo_amd as F
+    from fft_demo_amd import _lib
+
+    plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt=mid)
+    x = to_device(complex_input((batch, n), 23, "c16"), plan.in_dtype(), dev)
+    want = plan.exec(x)
+    got = plan.empty_out(device=dev)
+    got.zero_()
+    s = torch.cuda.current_stream()
+    lib = _lib.load()
+    npass = len(plan.info["passes"])
+    assert npass >= 2
+    for k in range(npass):
+        assert lib.mfft_exec_pass(plan._h, k, x.data_ptr(), got.data_ptr(), s.cuda_stream) == 0
+    assert lib.mfft_exec_pass(plan._h, npass, x.data_ptr(), got.data_ptr(), s.cuda_stream) == -1
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
