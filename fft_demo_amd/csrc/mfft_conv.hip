@@ -99,24 +99,41 @@ __global__ void pointwise_mac_kernel(const void *X, const float2 *H, void *Y, in
 }
 
 // out[2k][f][y][x] = scale * Re Y[k][f][y + oy][x + ox], out[2k+1][f][y][x] = scale * Im ...
-// (one thread per pair element: each spectrum value is read once for both images).
-// Workgroup b covers 256 columns of output row b / xb (xb workgroups per row), so the row
-// is decoded once per workgroup and a thread only adds its column.
-template <typename T, int FMT>
+// (each spectrum value is read once for both images). Workgroup b covers 256 * V columns of
+// output row b / xb (xb workgroups per row), so the row is decoded once per workgroup; a
+// thread takes V consecutive columns (V = 4: one 16-byte store per image, for fp32 output
+// rows whose width is a multiple of 4 on a 16-byte aligned buffer).
+template <typename T, int FMT, int V>
 __global__ void crop_unpack_kernel(const void *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
                                    int ox, float scale, int xb)
 {
     const unsigned row = blockIdx.x / (unsigned)xb;
-    const int x = (int)(blockIdx.x - row * (unsigned)xb) * 256 + threadIdx.x;
+    const int x = ((int)(blockIdx.x - row * (unsigned)xb) * 256 + threadIdx.x) * V;
     if (x >= ow) return;
     const unsigned kf = row / (unsigned)oh;
     const int y = (int)(row - kf * (unsigned)oh);
     const long long k = kf / (unsigned)F;
     const int f = (int)(kf - (unsigned)k * (unsigned)F);
-    const float2 v = spec_ld<FMT>(Y, ((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox));
+    const long long src = ((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox);
     const long long plane = (long long)F * oh * ow, off = ((long long)f * oh + y) * ow + x;
-    out[2 * k * plane + off] = (T)(v.x * scale);
-    if (2 * k + 1 < B) out[(2 * k + 1) * plane + off] = (T)(v.y * scale);
+    const bool second = 2 * k + 1 < B;
+    if constexpr (V == 1) {
+        const float2 v = spec_ld<FMT>(Y, src);
+        out[2 * k * plane + off] = (T)(v.x * scale);
+        if (second) out[(2 * k + 1) * plane + off] = (T)(v.y * scale);
+    } else {
+        static_assert(V == 4, "4 columns per thread");
+        float re[4], im[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float2 v = spec_ld<FMT>(Y, src + j);
+            re[j] = v.x * scale;
+            im[j] = v.y * scale;
+        }
+        *reinterpret_cast<float4 *>(out + 2 * k * plane + off) = make_float4(re[0], re[1], re[2], re[3]);
+        if (second)
+            *reinterpret_cast<float4 *>(out + (2 * k + 1) * plane + off) = make_float4(im[0], im[1], im[2], im[3]);
+    }
 }
 
 // Two real images a, b (separate buffers, [h][w]) -> one complex image a + i b, zero-padded.
@@ -304,29 +321,31 @@ hipError_t conv_pointwise(const void *X, const float2 *H, void *Y, int fmt, long
 }
 
 template <int FMT>
-static void launch_crop(const void *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy, int ox,
-                        float scale, int xb, unsigned blocks, hipStream_t s)
+static hipError_t launch_crop(const void *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
+                              int ox, float scale, hipStream_t s)
 {
+    const bool vec = !f64 && ow % 4 == 0 && ((uintptr_t)out & 15) == 0;
+    const int xb = (int)nblocks(vec ? (ow + 3) / 4 : ow);
+    const long long blocks = (long long)((B + 1) / 2) * F * oh * xb;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     if (f64)
-        hipLaunchKernelGGL((crop_unpack_kernel<double, FMT>), dim3(blocks), dim3(256), 0, s, Y, (double *)out, B, F, oh,
-                           ow, Ph, Pw, oy, ox, scale, xb);
+        hipLaunchKernelGGL((crop_unpack_kernel<double, FMT, 1>), dim3((unsigned)blocks), dim3(256), 0, s, Y,
+                           (double *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb);
+    else if (vec)
+        hipLaunchKernelGGL((crop_unpack_kernel<float, FMT, 4>), dim3((unsigned)blocks), dim3(256), 0, s, Y,
+                           (float *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb);
     else
-        hipLaunchKernelGGL((crop_unpack_kernel<float, FMT>), dim3(blocks), dim3(256), 0, s, Y, (float *)out, B, F, oh,
-                           ow, Ph, Pw, oy, ox, scale, xb);
+        hipLaunchKernelGGL((crop_unpack_kernel<float, FMT, 1>), dim3((unsigned)blocks), dim3(256), 0, s, Y,
+                           (float *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb);
+    return hipGetLastError();
 }
 
 hipError_t conv_crop(const void *Y, int fmt, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
                      int ox, float scale, hipStream_t s)
 {
-    const int xb = (int)nblocks(ow);
-    const long long blocks = (long long)((B + 1) / 2) * F * oh * xb;
-    if (blocks == 0) return hipSuccess;
-    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (fmt == FMT_C16)
-        launch_crop<FMT_C16>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, (unsigned)blocks, s);
-    else
-        launch_crop<FMT_C32>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, (unsigned)blocks, s);
-    return hipGetLastError();
+    if (fmt == FMT_C16) return launch_crop<FMT_C16>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, s);
+    return launch_crop<FMT_C32>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, s);
 }
 
 hipError_t conv_pointwise_gw(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph, int Pw,
