@@ -55,14 +55,39 @@ def flops(shape, batch):
     return 5.0 * n * math.log2(n) * batch
 
 
-def make_input(shape, batch, seed=42):
+def make_input(shape, batch, seed=42, spikes=False):
+    """Interleaved re/im uniform[-1,1] (SplitMix64), rounded to fp16. spikes: 10% of the
+    complex entries (mask from the next seed) scaled x100, to stress dynamic range."""
     import numpy as np
 
     import oracle
 
     n = math.prod(shape)
-    v = oracle.round_fp16(oracle.uniform(batch * n * 2, seed)).reshape(batch, *shape, 2)
-    return v
+    v = oracle.uniform(batch * n * 2, seed).reshape(batch * n, 2)
+    if spikes:
+        mask = oracle.uniform(batch * n, seed + 1, 0.0, 1.0) < 0.1
+        v[mask] *= 100.0
+    return oracle.round_fp16(v).reshape(batch, *shape, 2)
+
+
+def round_trip(shape, batch, xh, dev, sample):
+    """Forward (per-stage 1/radix) then inverse (unscaled) split-fp16 transforms with fp16
+    intermediates on `sample` items: (rel_l2, rel_max) of the round trip against the input."""
+    import numpy as np
+    import torch
+
+    import fft_demo_amd as F
+    import oracle
+
+    fwd = F.Plan(shape, sample, precision="split_fp16", mid_fmt="c16")
+    inv = F.Plan(shape, sample, precision="split_fp16", mid_fmt="c16", scale=0, direction=1)
+    x = torch.from_numpy(np.ascontiguousarray(xh[:sample])).to(device=dev, dtype=fwd.in_dtype()).contiguous()
+    back = inv.exec(fwd.exec(x)).float().cpu().numpy().astype(np.float64)
+    torch.cuda.synchronize()
+    l2, mx = oracle.err_l2max(back[..., 0] + 1j * back[..., 1], xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1])
+    fwd.close()
+    inv.close()
+    return l2, mx
 
 
 def time_plan(plan, x, out, steps, warmup, stream):
@@ -589,7 +614,8 @@ def main():
                           "config": {"workload": sv["workload"]}, "serve": sv}))
         return
     shape, batch, desc = CONFIGS[args.config]
-    xh = make_input(shape, batch)
+    spikes = args.config == "c3"  # the 2D workload carries 10% entries x100
+    xh = make_input(shape, batch, spikes=spikes)
     sample = 4 if len(shape) == 1 else 1
 
     # headline: split fp16 with fp16 intermediates between passes (one rounding per pass
@@ -645,7 +671,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp16",
-        "data": "synthetic: re/im uniform[-1,1] from SplitMix64 seed 42, rounded to fp16",
+        "data": "synthetic: re/im uniform[-1,1] from SplitMix64 seed 42" +
+                (", 10% of entries x100 (mask seed 43)" if spikes else "") + ", rounded to fp16",
         "config": {"workload": desc, "shape": list(shape), "batch": batch, "precision": "split_fp16 (hi+lo, fp32 acc)",
                    "io": "c16 in / c16 out", "intermediates": "c16 between passes",
                    "scale": "1/16 per radix-16 stage (out = DFT/N)"},
@@ -662,6 +689,10 @@ def main():
         "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3) if "fp32" in comps else None,
         "roofline": roofline,
     }
+    if len(shape) == 2:
+        rl2, rmx = round_trip(shape, batch, xh, dev, sample)
+        line["round_trip"] = {"rel_l2": rl2, "rel_max": rmx,
+                              "sample": f"{sample} transform(s): forward (1/radix per stage) then inverse, vs the input"}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(shape, batch, xh)
     if not args.no_serve:
