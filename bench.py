@@ -41,6 +41,7 @@ METRIC = "FFT GFLOP/s (5NlogN) at rel-L2≤1e-3 vs plain FP32 GPU FFT; serve p50
 HBM_PEAK_GBS = 8000.0
 TUNE = True
 TIMING = ""
+PASS_TIMING = ""
 
 CONFIGS = {
     # name: (shape, batch, description)
@@ -94,9 +95,12 @@ def time_plan(plan, x, out, steps, warmup, stream):
     """Time `steps` back-to-back execs between two HIP events on the exec stream; when an
     exec is shorter than 0.5 ms (the Python wrapper's launch overhead would then leave gaps
     in the queue) a hipGraph holding the `steps` execs is timed too and the faster is reported.
-    Each pass is then timed on its own: `steps` back-to-back launches of that pass alone
-    (mfft_exec_pass) between two HIP events on the same stream give its average launch time
-    (for the roofline). Returns (total_ms, [per-pass avg ms])."""
+    Each pass is then timed on its own (for the roofline): `steps` launches of that pass
+    alone (mfft_exec_pass), each between a start and a stop HIP event that the runtime
+    records around the kernel itself on the exec stream (its execution window, as a kernel
+    trace reports it); the average is the pass's launch time. Plans that run their batch in
+    several chunks fall back to two events around `steps` back-to-back launches of the pass
+    (launch gaps included). Returns (total_ms, [per-pass avg ms])."""
     import torch
 
     from fft_demo_amd.hiptime import EventPool
@@ -137,18 +141,31 @@ def time_plan(plan, x, out, steps, warmup, stream):
             TIMING = "hipGraph replay of the steps (captured execs), between two HIP events"
         del g
     ends.close()
+    global PASS_TIMING
     per_pass = []
-    ends = EventPool(2)
+    single = plan.info["batch_chunk"] >= plan.batch
+    PASS_TIMING = ("kernel window: start/stop HIP events recorded around each launch (hipExtLaunchKernel)"
+                   if single else "two HIP events around back-to-back launches of the pass")
+    pool = EventPool(2 * steps)
     for k in range(npass):
-        ends.record(0, stream.cuda_stream)
-        for _ in range(steps):
-            rc = lib.mfft_exec_pass(plan._h, k, x.data_ptr(), out.data_ptr(), stream.cuda_stream)
-            if rc != 0:
-                raise RuntimeError(lib.mfft_last_error().decode())
-        ends.record(1, stream.cuda_stream)
-        torch.cuda.synchronize()
-        per_pass.append(ends.elapsed_ms(0, 1) / steps)
-    ends.close()
+        if single:
+            for i in range(steps):
+                rc = lib.mfft_exec_pass(plan._h, k, x.data_ptr(), out.data_ptr(), stream.cuda_stream,
+                                        pool.arr[2 * i], pool.arr[2 * i + 1])
+                if rc != 0:
+                    raise RuntimeError(lib.mfft_last_error().decode())
+            torch.cuda.synchronize()
+            per_pass.append(sum(pool.elapsed_ms(2 * i, 2 * i + 1) for i in range(steps)) / steps)
+        else:
+            pool.record(0, stream.cuda_stream)
+            for _ in range(steps):
+                rc = lib.mfft_exec_pass(plan._h, k, x.data_ptr(), out.data_ptr(), stream.cuda_stream, None, None)
+                if rc != 0:
+                    raise RuntimeError(lib.mfft_last_error().decode())
+            pool.record(1, stream.cuda_stream)
+            torch.cuda.synchronize()
+            per_pass.append(pool.elapsed_ms(0, 1) / steps)
+    pool.close()
     # leave `out` holding a full exec's result
     plan.exec(x, out, stream=stream)
     torch.cuda.synchronize()
@@ -195,6 +212,7 @@ def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_
         "launches_per_exec": plan.info["launches_per_exec"],
         "schedule": sched,
         "timing": TIMING,
+        "pass_timing": PASS_TIMING,
     }
     del x, out
     plan.close()
@@ -682,6 +700,7 @@ def main():
         "whole_exec_hbm_gbs": round(whole_bytes / (head["ms_per_exec"] * 1e-3) / 1e9, 1),
         "passes": head["passes"],
         "pass_avg_ms": [round(v, 5) for v in pp],
+        "pass_timing": head["pass_timing"],
         "batch_chunk": head["batch_chunk"],
         "schedule": head["schedule"],
         "timing": head["timing"],

@@ -124,7 +124,7 @@ SIGNATURES = {
     "mfft_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(Info)]),
     "mfft_exec": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mfft_exec_events": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int]),
-    "mfft_exec_pass": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    "mfft_exec_pass": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "mfft_input_permutation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int]),
     "mfft_graph_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp]),
     "mfft_graph_launch": (ctypes.c_int, [_vp, _vp]),

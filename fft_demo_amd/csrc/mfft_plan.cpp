@@ -13,6 +13,8 @@
 #include "mfft_fast.h"
 #include "mfft_internal.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -29,7 +31,8 @@ hipError_t gen_dft_ops(DftOps *ops, int sign, hipStream_t s);
 typedef void (*tile_fn)(TileArgs);
 tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
 int pow2_tile_lines(int R, int TL);
-hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream);
+hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream, hipEvent_t start = nullptr,
+                       hipEvent_t stop = nullptr);
 const void *find_small_kernel(int R, int inf, int outf);
 hipError_t launch_queue(int prec, const TileArgs *jobs, const long long *first_tile, int njobs, long long total,
                         unsigned long long *counter, int grid, hipStream_t s);
@@ -501,8 +504,11 @@ static void *buf_ptr(int which, const void *in, void *out, void *work)
     return which == BUF_IN ? const_cast<void *>(in) : which == BUF_OUT ? out : work;
 }
 
+// only >= 0: run that pass alone; kstart/kstop: events the runtime records around that
+// pass's kernel (its own execution window; one launch per exec).
 static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s, hipEvent_t *ev,
-                      int &evi, void *work, const mfft::ExecFuse *fz = nullptr, long long b0 = 0, int only = -1)
+                      int &evi, void *work, const mfft::ExecFuse *fz = nullptr, long long b0 = 0, int only = -1,
+                      hipEvent_t kstart = nullptr, hipEvent_t kstop = nullptr)
 {
     int k = -1;
     for (auto &ps : p->passes) {
@@ -570,20 +576,23 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         if (ntiles <= 0) continue;
         if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
         if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
+        auto launch = [&](dim3 grid, dim3 block) -> hipError_t {
+            void *args[] = {&a};
+            if (kstart || kstop) return hipExtLaunchKernel(ps.fn, grid, block, args, 0, s, kstart, kstop, 0);
+            return hipLaunchKernel(ps.fn, grid, block, args, 0, s);
+        };
         if (ps.small) {
             long long nl = (long long)a.c0 * a.c1 * a.c2;
-            void *args[] = {&a};
-            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)((nl + 255) / 256)), dim3(256), args, 0, s));
+            HIP_TRY(launch(dim3((unsigned)((nl + 255) / 256)), dim3(256)));
             continue;
         }
         if (ps.fast_kind >= 0) {
             a.tw_lds = ps.tw_N && (1 << ps.tw_bits) <= kFastTw && ((ps.tw_N + (1 << ps.tw_bits) - 1) >> ps.tw_bits) <= kFastTw;
             long long grid = std::min(ntiles, ps.max_grid);
             if (grid > 8) grid -= grid % 8;
-            void *args[] = {&a};
-            HIP_TRY(hipLaunchKernel(ps.fn, dim3((unsigned)grid), dim3(fast_threads(p->d.precision, ps.R, ps.fast_kind)), args, 0, s));
+            HIP_TRY(launch(dim3((unsigned)grid), dim3(fast_threads(p->d.precision, ps.R, ps.fast_kind))));
         } else {
-            HIP_TRY(launch_tile(ps.fn, a, ntiles, s));
+            HIP_TRY(launch_tile(ps.fn, a, ntiles, s, kstart, kstop));
         }
     }
     return MFFT_OK;
@@ -755,7 +764,8 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
 }
 
 static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEvent_t *ev, long long batch = -1,
-                    void *work = nullptr, const mfft::ExecFuse *fz = nullptr, int only = -1)
+                    void *work = nullptr, const mfft::ExecFuse *fz = nullptr, int only = -1,
+                    hipEvent_t kstart = nullptr, hipEvent_t kstop = nullptr)
 {
     std::unique_lock<std::mutex> lk(p->exec_mu, std::defer_lock);
     if (!work) lk.lock();  // the plan's own workspace is shared by its callers
@@ -767,7 +777,7 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
     for (long long b0 = 0; b0 < batch; b0 += p->chunk) {
         long long nb = std::min(p->chunk, batch - b0);
         int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work,
-                            fz, b0, only);
+                            fz, b0, only, kstart, kstop);
         if (rc) return rc;
     }
     if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
@@ -920,12 +930,16 @@ int mfft_batcher_exec(mfft_batcher b, const mfft_batch_item *items, int nitems, 
     return MFFT_OK;
 }
 
-int mfft_exec_pass(mfft_plan p, int pass, const void *in, void *out, void *stream)
+int mfft_exec_pass(mfft_plan p, int pass, const void *in, void *out, void *stream, void *start_event, void *stop_event)
 {
     if (!p || !in || !out) return fail(MFFT_ERR_INVALID, "null argument");
     if (pass < 0 || pass >= (int)p->passes.size())
         return fail(MFFT_ERR_INVALID, "pass %d of %d", pass, (int)p->passes.size());
-    return exec_all(p, in, out, stream, nullptr, -1, nullptr, nullptr, pass);
+    if ((start_event || stop_event) && p->chunk < p->batch)
+        return fail(MFFT_ERR_UNSUPPORTED, "kernel events need a single-chunk plan (%lld per chunk, batch %lld)",
+                    p->chunk, p->batch);
+    return exec_all(p, in, out, stream, nullptr, -1, nullptr, nullptr, pass, (hipEvent_t)start_event,
+                    (hipEvent_t)stop_event);
 }
 
 int mfft_exec_events(mfft_plan p, const void *in, void *out, void *stream, void **events, int nevents)

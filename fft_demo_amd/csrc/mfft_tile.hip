@@ -16,6 +16,7 @@
 // half). Stages follow a self-sorting Stockham (decimation in frequency) schedule so
 // natural-order input gives natural-order output.
 #include "mfft_common.h"
+#include <hip/hip_ext.h>
 
 namespace mfft {
 
@@ -767,11 +768,13 @@ tile_fn find_tile_kernel(int prec, int R, int TL, bool generic)
     }
 }
 
-hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream)
+hipError_t launch_tile(const void *fn, const TileArgs &a, long long ntiles, hipStream_t stream, hipEvent_t start,
+                       hipEvent_t stop)
 {
     size_t lds = (size_t)tile_lds_bytes(a.R * a.T);
     dim3 grid((unsigned)ntiles), block(kThreads);
     void *args[] = {const_cast<TileArgs *>(&a)};
+    if (start || stop) return hipExtLaunchKernel(fn, grid, block, args, lds, stream, start, stop, 0);
     return hipLaunchKernel(fn, grid, block, args, lds, stream);
 }
 

@@ -135,9 +135,12 @@ int mfft_exec_events(mfft_plan plan, const void *in, void *out, void *stream, vo
 
 /* Runs pass `pass` (0 .. npasses-1) of an exec on its own, every chunk, on the buffers it
  * uses in a full exec (in, out or the plan's workspace): a pass reads what the previous
- * pass left. For per-kernel timing (back-to-back launches of one pass). No counterpart in
- * the reference. */
-int mfft_exec_pass(mfft_plan plan, int pass, const void *in, void *out, void *stream);
+ * pass left. start_event / stop_event (hipEvent_t, may be NULL): recorded by the runtime
+ * around the pass's kernel itself (hipExtLaunchKernel), i.e. its execution window without
+ * the launch gap; needs a plan that runs its batch in one chunk. For per-kernel timing.
+ * No counterpart in the reference. */
+int mfft_exec_pass(mfft_plan plan, int pass, const void *in, void *out, void *stream, void *start_event,
+                   void *stop_event);
 
 /* Work-queue batching: many transforms of uneven length (n <= 4096, radices
  * {16, 8, 4, 2, 3, 5, 17}) and batch count in ONE launch of a persistent kernel that takes

@@ -120,14 +120,17 @@ def test_short_line_kernels_all_precisions(dev, n, prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,batch,mid", [(65536, 8, "c16"), (1 << 20, 2, None)])
-def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid):
+@pytest.mark.parametrize("n,batch,mid,timed", [(65536, 8, "c16", True), (1 << 20, 2, None, False)])
+def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid, timed):
     """mfft_exec_pass run for every pass in order (the per-kernel timing entry point) leaves
-    exactly the output of one mfft_exec (bit for bit), on a two-pass and a longer plan."""
+    exactly the output of one mfft_exec (bit for bit), on a two-pass and a longer plan; with
+    start/stop events (kernel launched through hipExtLaunchKernel) each pass's window is a
+    positive time."""
     import torch
 
     import fft_demo_amd as F
     from fft_demo_amd import _lib
+    from fft_demo_amd.hiptime import EventPool
 
     plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt=mid)
     x = to_device(complex_input((batch, n), 23, "c16"), plan.in_dtype(), dev)
@@ -138,8 +141,15 @@ def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid):
     lib = _lib.load()
     npass = len(plan.info["passes"])
     assert npass >= 2
+    pool = EventPool(2 * npass)
     for k in range(npass):
-        assert lib.mfft_exec_pass(plan._h, k, x.data_ptr(), got.data_ptr(), s.cuda_stream) == 0
-    assert lib.mfft_exec_pass(plan._h, npass, x.data_ptr(), got.data_ptr(), s.cuda_stream) == -1
+        ev = (pool.arr[2 * k], pool.arr[2 * k + 1]) if timed else (None, None)
+        assert lib.mfft_exec_pass(plan._h, k, x.data_ptr(), got.data_ptr(), s.cuda_stream, *ev) == 0
+    assert lib.mfft_exec_pass(plan._h, npass, x.data_ptr(), got.data_ptr(), s.cuda_stream, None, None) == -1
     torch.cuda.synchronize()
     assert torch.equal(got, want)
+    if timed:
+        ms = [pool.elapsed_ms(2 * k, 2 * k + 1) for k in range(npass)]
+        print(f"pass windows (ms): {ms}")
+        assert all(0 < t < 100 for t in ms)
+    pool.close()
