@@ -191,10 +191,14 @@ def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_
     from fft_demo_amd import tune
 
     prec = name
+    # matrix-core variants scale 1/radix per stage (fp16/bf16 would overflow on the 2D
+    # workload's x100 entries otherwise); plain fp32 is unscaled
+    scale = 0 if prec == "fp32" else 1
     # every variant gets the same treatment: autotuned schedule (split point, batch chunk),
     # unless --no-tune (profiling runs: only the planner's schedule is launched)
-    sched = tune.autotune(shape, batch, precision=prec, mid_fmt=mid_fmt) if TUNE else None
-    plan = F.Plan(shape, batch, precision=prec, split_hint=tune.SPLIT_TUNED if TUNE else 0, mid_fmt=mid_fmt)
+    sched = tune.autotune(shape, batch, precision=prec, mid_fmt=mid_fmt, scale=scale) if TUNE else None
+    plan = F.Plan(shape, batch, precision=prec, split_hint=tune.SPLIT_TUNED if TUNE else 0, mid_fmt=mid_fmt,
+                  scale=scale)
     dt = plan.in_dtype()
     x = torch.from_numpy(xh).to(device=dev, dtype=dt).contiguous()
     out = plan.empty_out(device=dev)
