@@ -260,11 +260,10 @@ struct Prefetch {
     static constexpr int N = PREC == PREC_FP32 ? 16 : 4 * G::MAXP;
     raw w[N];
 
-    // fp32: full tiles (every line valid, the common case) load without per-lane masks
-    // (faster there; the matrix-core kernels measured faster with the masked form).
+    // Full tiles (every line valid, the common case) load without per-lane masks.
     __device__ __forceinline__ void load(const TileArgs &a, const Origin &o)
     {
-        if (PREC == PREC_FP32 && o.nv == G::T)
+        if (o.nv == G::T)
             load_lines<true>(a, o);
         else
             load_lines<false>(a, o);
@@ -678,6 +677,10 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const
     walk.init(a, G::T, tile < end ? tile : 0, step);
     Origin o = walk.origin(a, G::T);
     if (tile < end) cur.load(a, o);
+    // the loop starts with no loads in flight: otherwise the first tile's pending loads
+    // make every iteration wait for all loads (its own prefetch included) before its
+    // first product
+    wait_prefetch();
     while (tile < end) {
         const long long next = tile + step;
         Origin on = o;
