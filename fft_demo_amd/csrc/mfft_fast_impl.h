@@ -260,17 +260,10 @@ struct Prefetch {
     static constexpr int N = PREC == PREC_FP32 ? 16 : 4 * G::MAXP;
     raw w[N];
 
-    // Full tiles (every line valid, the common case) load without per-lane masks.
-    __device__ __forceinline__ void load(const TileArgs &a, const Origin &o)
-    {
-        if (o.nv == G::T)
-            load_lines<true>(a, o);
-        else
-            load_lines<false>(a, o);
-    }
-
+    // FULL: every tile of the launch has all its lines (the line count is a multiple of
+    // the tile's), so the loads need no per-lane masks.
     template <bool FULL>
-    __device__ __forceinline__ void load_lines(const TileArgs &a, const Origin &o)
+    __device__ __forceinline__ void load(const TileArgs &a, const Origin &o)
     {
         constexpr int MAP = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
         constexpr int R16 = G::R16;
@@ -645,9 +638,11 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
 
 // ------------------------------------------------------------------ kernel
 // The persistent tile loop. TW: the last stage multiplies by the four-step twiddles
-// (a.tw_N != 0); the loop is instantiated once per case so neither epilogue carries the
-// other's registers or selects.
-template <int PREC, int R, int KIND, int INF, int OUTF, bool TW>
+// (a.tw_N != 0); FULL: every tile is full (unmasked loads). The loop is instantiated once
+// per case so no variant carries another's registers, selects or branches (a masked and
+// an unmasked load path in one loop made the compiler wait for every outstanding memory
+// operation, the previous tile's stores included, before each tile's loads).
+template <int PREC, int R, int KIND, int INF, int OUTF, bool TW, bool FULL>
 __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const float2 *stw, const float2 *s_tw,
                                            const Ops<PREC> &op)
 {
@@ -676,7 +671,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const
     TileWalk walk;
     walk.init(a, G::T, tile < end ? tile : 0, step);
     Origin o = walk.origin(a, G::T);
-    if (tile < end) cur.load(a, o);
+    if (tile < end) cur.template load<FULL>(a, o);
     // the loop starts with no loads in flight: otherwise the first tile's pending loads
     // make every iteration wait for all loads (its own prefetch included) before its
     // first product
@@ -687,7 +682,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const
         if (next < end) {
             walk.advance(a);
             on = walk.origin(a, G::T);
-            nxt.load(a, on);
+            nxt.template load<FULL>(a, on);
         }
         int s = 1;
         if constexpr (PREC == PREC_FP32)
@@ -755,10 +750,18 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
     if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
     __syncthreads();
 
-    if (a.tw_N)
-        fast_tiles<PREC, R, KIND, INF, OUTF, true>(a, lds, stw, s_tw, op);
-    else
-        fast_tiles<PREC, R, KIND, INF, OUTF, false>(a, lds, stw, s_tw, op);
+    const bool full = a.c0 % G::T == 0;
+    if (a.tw_N) {
+        if (full)
+            fast_tiles<PREC, R, KIND, INF, OUTF, true, true>(a, lds, stw, s_tw, op);
+        else
+            fast_tiles<PREC, R, KIND, INF, OUTF, true, false>(a, lds, stw, s_tw, op);
+    } else {
+        if (full)
+            fast_tiles<PREC, R, KIND, INF, OUTF, false, true>(a, lds, stw, s_tw, op);
+        else
+            fast_tiles<PREC, R, KIND, INF, OUTF, false, false>(a, lds, stw, s_tw, op);
+    }
 }
 
 template <int PREC, int KIND, int INF, int OUTF>
