@@ -77,8 +77,9 @@ struct Geo {
         else
             return t * LP + p + (p >> 4);
     }
-    // twiddle tables staged in LDS (0 = read from global memory)
-    static constexpr int STW = R <= 1024 ? R : 0;
+    // stage twiddles staged in LDS (R entries, 32 KB at R = 4096): a read from global
+    // memory inside the tile loop would make the wave wait for the next tile's prefetch
+    static constexpr int STW = R;
     static constexpr int PAIRS = R * T / 256;               // MFMA column groups per stage
     static constexpr int MAXP = (PAIRS + NW - 1) / NW;
 };
@@ -545,8 +546,11 @@ __device__ __forceinline__ void dft_tail(float2 *v, int sign)
     }
 }
 
+// stw: the stage twiddles with the matrix-core scale a.last_mul folded in (the table the
+// radix-16 stages use, staged in LDS); this VALU stage scales by `mul` instead, so its
+// products take mul / last_mul (a power of two) on the input side.
 template <int R, int KIND, int RADIX, int NW>
-__device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, float mul)
+__device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, float mul, const float2 *stw)
 {
     typedef Geo<R, KIND, NW> G;
     constexpr int T = G::T;
@@ -573,10 +577,11 @@ __device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, fl
         const int col = KIND == KIND_COLS ? I / T : I % COLS;
         dft_tail<RADIX>(v[k], a.sign);
         const int q = col % s, p = col / s;
+        const float comp = mul / a.last_mul;
 #pragma unroll
         for (int tq = 0; tq < RADIX; ++tq) {
             float2 y = make_float2(v[k][tq].x * mul, v[k][tq].y * mul);
-            if (p) y = cmulf(y, a.stage_tw[p * tq * s]);
+            if (p) y = cmulf(make_float2(v[k][tq].x * comp, v[k][tq].y * comp), stw[p * tq * s]);
             lds[G::phys(t, q + s * (RADIX * p + tq))] = y;
         }
     }
@@ -695,7 +700,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const
             tstage_last<R, KIND, TAIL, G::NW, OUTF, TW>(lds, a, o, s_tw);
         } else {
             if constexpr (TAIL > 1) {
-                tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul);
+                tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul, stw);
                 s *= TAIL;
             }
 #pragma unroll
