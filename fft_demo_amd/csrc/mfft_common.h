@@ -28,7 +28,9 @@ constexpr int kThreads = kWaves * 64;
 constexpr int kMaxTile = 4096;            // complex points per workgroup tile
 constexpr int kMaxPairsPerWave = kMaxTile / 256 / kWaves;  // MFMA column groups per wave/stage
 constexpr int kMaxValuPerThread = 32;
-constexpr int kFastTw = 256;              // four-step table entries staged in LDS (each half)     // complex values a thread holds in a VALU stage
+constexpr int kFastTw = 256;              // four-step table entries staged in LDS (each half)
+constexpr int kFastTwCols = 1024;         // the same for the strided kind (the four-step's first
+                                          // pass, where the twiddles are applied; N <= 2^20)
 
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
 __host__ __device__ constexpr int tile_lds_bytes(int points) { return lds_pad(points) * 8 + 64; }
@@ -65,7 +67,8 @@ struct TileArgs {
     const float2 *tw_lo_s;     // tw_lo * last-stage scale
     float last_mul;            // last-stage scale when there is no pass twiddle
     float tail_mul;            // scale of the radix-2/4/8 stage
-    int tw_lds;                // four-step tables fit in LDS (2^tw_bits and the hi part <= kFastTw)
+    int tw_lds;                // four-step tables fit in LDS (2^tw_bits and the hi part <= kFastTw,
+                               // kFastTwCols for the strided kind)
     // fused first-pass input (streaming kernel only; zero/null otherwise), element e of the
     // chunk = transform batch_base + e / real_p, offset e % real_p inside it:
     //   in_mul: the loaded value is multiplied by in_mul[e % in_mul_period]

@@ -80,6 +80,8 @@ struct Geo {
     // stage twiddles staged in LDS (R entries, 32 KB at R = 4096): a read from global
     // memory inside the tile loop would make the wave wait for the next tile's prefetch
     static constexpr int STW = R;
+    // four-step table entries staged in LDS per half (lo at [0, TWC), hi at [TWC, 2 TWC))
+    static constexpr int TWC = KIND == KIND_COLS ? kFastTwCols : kFastTw;
     static constexpr int PAIRS = R * T / 256;               // MFMA column groups per stage
     static constexpr int MAXP = (PAIRS + NW - 1) / NW;
 };
@@ -306,7 +308,7 @@ struct Prefetch {
 };
 
 // Four-step twiddles W_N^(mult * (pt0 + k * STEP)), k = 0..3, read before any store.
-template <int STEP, bool SCALED = true>
+template <int STEP, int TWC, bool SCALED = true>
 __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsigned mult, int pt0, float2 *w)
 {
     const unsigned mask = (1u << a.tw_bits) - 1u;
@@ -318,7 +320,7 @@ __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsign
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const unsigned e = mult * (unsigned)(pt0 + k * STEP);
-            const unsigned ih = 2 * (kFastTw + (e >> a.tw_bits)), il = 2 * (e & mask);
+            const unsigned ih = 2 * (TWC + (e >> a.tw_bits)), il = 2 * (e & mask);
             w[k] = cmulf(make_float2(t[ih], t[ih + 1]), make_float2(t[il], t[il + 1]));
         }
     } else {
@@ -332,11 +334,28 @@ __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsign
     }
 }
 
+// One four-step twiddle W_N^e (unscaled table), from LDS or global memory as in tw4.
+template <int TWC>
+__device__ __forceinline__ float2 tw1(const TileArgs &a, const float2 *twl, unsigned e)
+{
+    const unsigned mask = (1u << a.tw_bits) - 1u;
+    typedef __attribute__((address_space(3))) const float lds_f;
+    typedef __attribute__((address_space(1))) const float glb_f;
+    if (a.tw_lds) {
+        const lds_f *t = (const lds_f *)twl;
+        const unsigned ih = 2 * (TWC + (e >> a.tw_bits)), il = 2 * (e & mask);
+        return cmulf(make_float2(t[ih], t[ih + 1]), make_float2(t[il], t[il + 1]));
+    }
+    const glb_f *th = (const glb_f *)a.tw_hi, *tl = (const glb_f *)a.tw_lo;
+    const unsigned ih = 2 * (e >> a.tw_bits), il = 2 * (e & mask);
+    return cmulf(make_float2(th[ih], th[ih + 1]), make_float2(tl[il], tl[il + 1]));
+}
+
 // ------------------------------------------------------------------ radix-16 stage (MFMA)
 // FIRST: operands are the prefetched raw inputs; LAST: results go to global memory,
 // multiplied by the four-step twiddles when TW (a.tw_N != 0), else by a.last_mul.
 // stw: stage twiddles (LDS or global, scale folded in); twl: the four-step tables staged
-// in LDS (lo at [0, kFastTw), hi at [kFastTw, 2 kFastTw)) when a.tw_lds, else they are read
+// in LDS (lo at [0, TWC), hi at [TWC, 2 TWC)) when a.tw_lds, else they are read
 // from global memory (a.tw_lo_s, a.tw_hi). The two sources are separate code paths so
 // each keeps its address space (LDS reads wait on the LDS counter only, never on the
 // next tile's prefetch loads).
@@ -399,7 +418,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
                 typedef typename Elem<OUTF>::raw raw;
                 raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
                 float2 w[4];
-                if constexpr (TW) tw4<R16>(a, twl, (unsigned)(a.tw_level ? o.l1 : o.l0b + t), col + R16 * 4 * g, w);
+                if constexpr (TW) tw4<R16, G::TWC>(a, twl, (unsigned)(a.tw_level ? o.l1 : o.l0b + t), col + R16 * 4 * g, w);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int pt = col + R16 * (4 * g + i);
@@ -496,7 +515,7 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
             float2 w[4];
-            if constexpr (TW) tw4<4 * R16>(a, twl, mult, col + R16 * q4, w);
+            if constexpr (TW) tw4<4 * R16, G::TWC>(a, twl, mult, col + R16 * q4, w);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int tq = q4 + 4 * k;
@@ -621,7 +640,7 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
         for (int q4 = 0; q4 < RADIX; q4 += 4) {
             float2 w[4];
             if constexpr (TW && RADIX >= 4)
-                tw4<COLS, false>(a, twl, mult, col + COLS * q4, w);
+                tw4<COLS, G::TWC, false>(a, twl, mult, col + COLS * q4, w);
 #pragma unroll
             for (int kk = 0; kk < (RADIX < 4 ? RADIX : 4); ++kk) {
                 const int tq = q4 + kk;
@@ -631,8 +650,7 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
                     if constexpr (RADIX >= 4) {
                         y = cmulf(y, w[kk]);
                     } else {
-                        const unsigned e = mult * (unsigned)pt, mask = (1u << a.tw_bits) - 1u;
-                        y = cmulf(y, cmulf(a.tw_hi[e >> a.tw_bits], a.tw_lo[e & mask]));
+                        y = cmulf(y, tw1<G::TWC>(a, twl, mult * (unsigned)pt));
                     }
                 }
                 dst[(long long)pt * a.out_ps] = Elem<OUTF>::pack(y);
@@ -732,7 +750,7 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
     typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
     __shared__ float2 lds[G::SIZE];
     __shared__ float2 s_stw[G::STW > 0 ? G::STW : 1];
-    __shared__ float2 s_tw[2 * kFastTw];
+    __shared__ float2 s_tw[2 * G::TWC];
     constexpr int L2 = __builtin_ctz(R);
     constexpr int NS = L2 / 4;
     constexpr int TAIL = 1 << (L2 % 4);
@@ -749,7 +767,7 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
         // the tail-as-last stage (NS == 1) is VALU: it takes the unscaled lo table
         const float2 *lo = NS == 1 ? a.tw_lo : a.tw_lo_s;
         for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = lo[i];
-        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[kFastTw + i] = a.tw_hi[i];
+        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[G::TWC + i] = a.tw_hi[i];
     }
     Ops<PREC> op;
     if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);

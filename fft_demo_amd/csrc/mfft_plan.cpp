@@ -587,7 +587,8 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             continue;
         }
         if (ps.fast_kind >= 0) {
-            a.tw_lds = ps.tw_N && (1 << ps.tw_bits) <= kFastTw && ((ps.tw_N + (1 << ps.tw_bits) - 1) >> ps.tw_bits) <= kFastTw;
+            const int twc = ps.fast_kind == KIND_COLS ? kFastTwCols : kFastTw;
+            a.tw_lds = ps.tw_N && (1 << ps.tw_bits) <= twc && ((ps.tw_N + (1 << ps.tw_bits) - 1) >> ps.tw_bits) <= twc;
             long long grid = std::min(ntiles, ps.max_grid);
             if (grid > 8) grid -= grid % 8;
             HIP_TRY(launch(dim3((unsigned)grid), dim3(fast_threads(p->d.precision, ps.R, ps.fast_kind))));
