@@ -200,8 +200,16 @@ static long long odd_part(long long L)
 // Odd factors the streaming kernel takes as a pass of its own.
 static bool small_odd(long long o) { return o == 3 || o == 5 || o == 9 || o == 15 || o == 17; }
 
+// Whether a plan's intermediates between passes are 16-bit (the rule finalize_plan applies).
+static bool narrow_mid(const mfft_desc *d)
+{
+    if (d->mid_fmt >= 0) return d->mid_fmt != MFFT_FMT_C32;
+    return d->precision == MFFT_PREC_FP16 || d->precision == MFFT_PREC_BF16;
+}
+
+// narrow: the intermediates between passes are 16-bit (an extra pass costs 8 bytes a point)
 static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, long long ps_out,
-                     std::vector<Level> lines, int src, int dst, int perm_axis, int split_hint)
+                     std::vector<Level> lines, int src, int dst, int perm_axis, int split_hint, bool narrow)
 {
     // odd * 2^a (2176 = 17 x 128): the odd factor in a streaming pass, the power of two on
     // the matrix-core kernels, instead of one generic mixed-radix tile pass.
@@ -248,6 +256,11 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
         int m1 = (m + 1) / 2;
         if (m1 > 8) m1 = 8;
         if (m - m1 > 12 && m1 < 8) m1 = std::min(8, m - 12);
+        // 2^19..2^21 with 16-bit intermediates: a second pass of 2048 or 4096 points would
+        // hold one or two lines per tile and store them transposed in 4- or 8-byte pieces;
+        // three passes instead, the rest (8192 = 128 x 64) split again below (split fp16:
+        // 1.05-1.3x faster; with fp32 intermediates the third pass costs more than it saves)
+        if (narrow && m >= 19 && m <= 21) m1 = m - 13;
         L1 = 1LL << m1;
     } else {
         for (long long c = 256; c >= 2; --c)
@@ -276,7 +289,7 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     std::vector<Level> sub2;
     for (auto &l : lines) sub2.push_back(Level{l.count, l.in_stride, l.out_stride, l.batch});
     sub2.push_back(Level{L1, L2 * ps_in, ps_out, false});
-    return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, a_dst, dst, -1, 1);
+    return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, a_dst, dst, -1, 1, narrow);
 }
 
 // Choose tile geometry and kernel for every pass.
@@ -654,7 +667,7 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     }
     std::vector<Level> lines{Level{batch, n, n, true}};
     rc = plan_axis(p->passes, n, 1, 1, lines, BUF_IN, BUF_OUT,
-                   d->in_order == MFFT_ORDER_DIGIT_REVERSED ? 0 : -1, d->split_hint);
+                   d->in_order == MFFT_ORDER_DIGIT_REVERSED ? 0 : -1, d->split_hint, narrow_mid(d));
     if (!rc) rc = finalize_plan(p);
     if (rc) {
         std::string msg = g_err;
@@ -695,11 +708,11 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
     // rows (y axis, contiguous): lines = (x, b)
     std::vector<Level> rows{Level{nx, ny, ny, false}, Level{batch, NN, NN, true}};
     const int odd_hint = d->split_hint == 1 ? 1 : 0;  // 1 keeps odd * 2^a axes in one pass
-    rc = plan_axis(p->passes, ny, 1, 1, rows, BUF_IN, BUF_WORK, rev ? 1 : -1, odd_hint);
+    rc = plan_axis(p->passes, ny, 1, 1, rows, BUF_IN, BUF_WORK, rev ? 1 : -1, odd_hint, narrow_mid(d));
     // columns (x axis, stride ny): lines = (y, b)
     if (!rc) {
         std::vector<Level> cols{Level{ny, 1, 1, false}, Level{batch, NN, NN, true}};
-        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, odd_hint);
+        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, odd_hint, narrow_mid(d));
     }
     if (!rc) rc = finalize_plan(p);
     if (rc) {

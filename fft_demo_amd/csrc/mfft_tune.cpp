@@ -134,13 +134,13 @@ int mfft_autotune(int nx, int ny, int64_t batch, const mfft_desc *d, int reps, m
     const bool two_d = ny > 0;
     const long long n = two_d ? (long long)nx * ny : nx;
     // candidate split points (1D): the planner's choice, every power-of-two first pass of a
-    // power-of-two length whose second pass fits one tile, and for odd * 2^a lengths the
-    // single mixed-radix pass besides the odd-factor split
+    // power-of-two length (the rest in one pass when it fits a tile, else split again), and
+    // for odd * 2^a lengths the single mixed-radix pass besides the odd-factor split
     std::vector<int> splits{0};
     if (!two_d && d->in_order == MFFT_ORDER_NATURAL) {
         if (is_pow2(n) && n > 4096)
             for (int l1 = 2; l1 <= 256; l1 *= 2)
-                if (n / l1 <= 4096 && n % l1 == 0) splits.push_back(l1);
+                if (n / l1 >= 16 && n % l1 == 0) splits.push_back(l1);
         if (odd_part(n) > 1) splits.push_back(1);
     }
     if (two_d && odd_part(nx) * odd_part(ny) > 1 && nx <= 4096 && ny <= 4096) splits.push_back(1);

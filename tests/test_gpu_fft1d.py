@@ -153,3 +153,17 @@ def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid, timed):
         print(f"pass windows (ms): {ms}")
         assert all(0 < t < 100 for t in ms)
     pool.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1 << 19, 1 << 20, 1 << 21])
+@pytest.mark.parametrize("prec,mid", [("split_fp16", "c16"), ("fp16", None)])
+def test_three_pass_lengths_narrow_intermediates(dev, n, prec, mid):
+    """2^19..2^21 with 16-bit intermediates plan three passes (first pass 2^(m-13), then
+    8192 = 128 x 64); tolerance as for the precision."""
+    y, ref, plan = _run(dev, n, 2, prec, mid_fmt=mid)
+    l2, mx = rel_errors(y, ref)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    print(f"n=2^{n.bit_length() - 1} {prec} mid={mid}: rel_l2={l2:.2e} max={mx:.2e} passes={kinds}")
+    assert len(kinds) == 3
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
