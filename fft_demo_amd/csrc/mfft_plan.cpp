@@ -246,9 +246,14 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
         L1 = odd;
     } else if (split_hint > 1 && L % split_hint == 0 && split_hint <= 256 && factorable(split_hint)) {
         L1 = split_hint;
+    } else if (is_pow2(L) && ps_in != 1 && L >= 512 && L <= 4096) {
+        // strided axis (2D columns) of 512..4096 points: both passes are strided; the first
+        // on the matrix-core kernel (L / 16 points, at least 64), the rest (16 or fewer) on
+        // the streaming kernel (2048 x 2048: 1.1x the 16 + 128 split, measured)
+        L1 = std::max<long long>(64, L / 16);
     } else if (is_pow2(L) && ps_in != 1 && L % 256 == 0 && L / 256 <= 256) {
-        // strided axis (2D columns): both passes are strided; give the second the
-        // 256-point matrix-core kernel and the small remainder to the first
+        // longer strided axes: give the second pass the 256-point matrix-core kernel and the
+        // small remainder to the first
         L1 = L / 256;
     } else if (is_pow2(L)) {
         int m = 0;
@@ -711,8 +716,10 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
     rc = plan_axis(p->passes, ny, 1, 1, rows, BUF_IN, BUF_WORK, rev ? 1 : -1, odd_hint, narrow_mid(d));
     // columns (x axis, stride ny): lines = (y, b)
     if (!rc) {
+        // split_hint > 1: the first-pass length of the column axis when it is split
+        const int col_hint = d->split_hint > 1 ? d->split_hint : odd_hint;
         std::vector<Level> cols{Level{ny, 1, 1, false}, Level{batch, NN, NN, true}};
-        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, odd_hint, narrow_mid(d));
+        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, col_hint, narrow_mid(d));
     }
     if (!rc) rc = finalize_plan(p);
     if (rc) {
