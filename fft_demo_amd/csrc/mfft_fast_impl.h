@@ -334,8 +334,10 @@ __device__ __forceinline__ void tw4(const TileArgs &a, const float2 *twl, unsign
     }
 }
 
-// One four-step twiddle W_N^e (unscaled table), from LDS or global memory as in tw4.
-template <int TWC>
+// One four-step twiddle W_N^e, from LDS (the staged tables) or global memory as in tw4;
+// SCALED: the stage scale folded into the lo table (the global path picks the table, the
+// LDS path holds whichever the kernel staged).
+template <int TWC, bool SCALED = false>
 __device__ __forceinline__ float2 tw1(const TileArgs &a, const float2 *twl, unsigned e)
 {
     const unsigned mask = (1u << a.tw_bits) - 1u;
@@ -346,7 +348,7 @@ __device__ __forceinline__ float2 tw1(const TileArgs &a, const float2 *twl, unsi
         const unsigned ih = 2 * (TWC + (e >> a.tw_bits)), il = 2 * (e & mask);
         return cmulf(make_float2(t[ih], t[ih + 1]), make_float2(t[il], t[il + 1]));
     }
-    const glb_f *th = (const glb_f *)a.tw_hi, *tl = (const glb_f *)a.tw_lo;
+    const glb_f *th = (const glb_f *)a.tw_hi, *tl = (const glb_f *)(SCALED ? a.tw_lo_s : a.tw_lo);
     const unsigned ih = 2 * (e >> a.tw_bits), il = 2 * (e & mask);
     return cmulf(make_float2(th[ih], th[ih + 1]), make_float2(tl[il], tl[il + 1]));
 }
@@ -397,6 +399,21 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
     }
     if constexpr (!FIRST && !LAST) __syncthreads();  // all reads of the tile precede the writes
     if constexpr (LAST) wait_prefetch();
+    // Last stage with the four-step twiddle: the lane's outputs of group u are the points
+    // col + R16 (4g + i), i = 0..3, so their twiddles are w_i = W^(mult (col + 4g R16)) *
+    // s^i with s = W^(mult R16): one table lookup for w_0, products for the rest. When the
+    // lane's line is the same for every group, s is looked up once per tile.
+    constexpr bool SAME_T = MAP == MAP_LINES && (16 * G::NW) % T == 0;
+    float2 tw_step = make_float2(0.f, 0.f);
+    if constexpr (LAST && TW && SAME_T) {
+        int t, col;
+        slot_tc<R, T, MAP>(wave * 16 + c, t, col);
+        const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+        tw_step = tw1<G::TWC, true>(a, twl, mult * (unsigned)R16);
+        const float inv = 1.0f / a.last_mul;  // the table's stage scale, a power of two
+        tw_step.x *= inv;
+        tw_step.y *= inv;
+    }
 #pragma unroll
     for (int u = 0; u < G::MAXP; ++u) {
         const int P = wave + u * G::NW;
@@ -418,7 +435,20 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
                 typedef typename Elem<OUTF>::raw raw;
                 raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
                 float2 w[4];
-                if constexpr (TW) tw4<R16, G::TWC>(a, twl, (unsigned)(a.tw_level ? o.l1 : o.l0b + t), col + R16 * 4 * g, w);
+                if constexpr (TW) {
+                    const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+                    float2 st = tw_step;
+                    if constexpr (!SAME_T) {
+                        st = tw1<G::TWC, true>(a, twl, mult * (unsigned)R16);
+                        const float inv = 1.0f / a.last_mul;
+                        st.x *= inv;
+                        st.y *= inv;
+                    }
+                    w[0] = tw1<G::TWC, true>(a, twl, mult * (unsigned)(col + R16 * 4 * g));
+                    w[1] = cmulf(w[0], st);
+                    w[2] = cmulf(w[1], st);
+                    w[3] = cmulf(w[2], st);
+                }
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int pt = col + R16 * (4 * g + i);
