@@ -402,7 +402,7 @@ def _rocfft_plain(trace, filters, dev):
 
 
 def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, log):
-    """Highest arrival rate (x1.5 steps from start_rate, then two bisection steps) whose
+    """Highest arrival rate (x1.5 steps from start_rate, then four bisection steps) whose
     p99 stays within the bound;
     returns the replay result at that rate. shapes_trace covers the request shapes (plans
     and operators are built from it before the probes)."""
@@ -428,9 +428,10 @@ def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, 
                 break
             best, lo = r, rate
             rate *= 1.5
-        # two geometric bisection steps between the last rate that held and the first that did not
+        # four geometric bisection steps between the last rate that held and the first that
+        # did not (resolves the knee to about 2.5%)
         if best is not None and fail is not None:
-            for _ in range(2):
+            for _ in range(4):
                 mid = (lo * fail) ** 0.5
                 r, ok = probe(mid)
                 if ok:
