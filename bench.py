@@ -416,7 +416,8 @@ def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, 
     serve.warm(S, shapes_trace, ar)
     def probe(rate):
         r = serve.replay(S, serve.make_trace(rate, seconds, 43), ar)
-        log.append({"rate": round(rate, 1), "p99_ms": round(r["p99_ms"], 3), "req_per_s": round(r["req_per_s"], 1)})
+        log.append({"rate": round(rate, 1), "p99_ms": round(r["p99_ms"], 3), "req_per_s": round(r["req_per_s"], 1),
+                    "submit_lag_max_ms": round(r["submit_lag_max_ms"], 3)})
         return r, r["p99_ms"] <= target_p99_ms and r["req_per_s"] >= 0.9 * rate
 
     try:
@@ -467,6 +468,7 @@ def serve_section(seconds, sat_requests=2000, p99_bound_ms=2.0):
         del ar
         torch.cuda.empty_cache()
         out[name] = {"p50_ms": round(r["p50_ms"], 4), "p99_ms": round(r["p99_ms"], 4),
+                     "submit_lag_max_ms": round(r["submit_lag_max_ms"], 3), "scheduler_busy_s": st["busy_us"] / 1e6,
                      "req_per_s": round(r["req_per_s"], 1), "gflops": round(r["gflops"], 1),
                      "saturation_req_per_s": round(sat["req_per_s"], 1),
                      "saturation_gflops": round(sat["gflops"], 1),

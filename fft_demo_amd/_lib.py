@@ -216,7 +216,7 @@ class Completion(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("coalesced", ctypes.c_int64), ("static_bytes", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64),
-                ("peak_bytes", ctypes.c_int64), ("paired", ctypes.c_int64)]
+                ("peak_bytes", ctypes.c_int64), ("paired", ctypes.c_int64), ("busy_us", ctypes.c_int64)]
 
 
 REQ_FFT1D, REQ_FFT2D, REQ_CONV2D = 1, 2, 3

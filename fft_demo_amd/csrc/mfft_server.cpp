@@ -93,8 +93,11 @@ struct mfft_server_s {
     std::vector<Req *> inflight;
     std::vector<mfft_completion> done;
     std::atomic<long long> used{0};  // request I/O of admitted, unfinished requests
-    long long peak = 0;              // largest static_bytes + used seen at an admission
-    long long submitted = 0, completed = 0, launches = 0, coalesced = 0, paired = 0;
+    std::atomic<long long> peak{0};  // largest static_bytes + used seen at an admission
+    long long submitted = 0, completed = 0;  // under mu
+    // written by the scheduler thread, read by mfft_server_get_stats
+    std::atomic<long long> launches{0}, coalesced{0}, paired{0};
+    std::atomic<long long> busy_ns{0};  // scheduler time spent retiring, admitting, launching
     std::atomic<bool> stop{false};
     int device = 0;
     std::thread sched;
@@ -375,6 +378,7 @@ void scheduler(mfft_server_s *S)
                 S->incoming.pop_front();
             }
         }
+        const auto t_busy = std::chrono::steady_clock::now();
         // retire: query each event once; requests of one coalesced or paired launch share
         // their event and retire together (its owner returns it to the pool)
         {
@@ -414,7 +418,7 @@ void scheduler(mfft_server_s *S)
             q->t_start = now_us();
             q->admitted = true;
             S->used += q->need;
-            S->peak = std::max(S->peak, S->static_bytes + S->used);
+            S->peak = std::max(S->peak.load(), S->static_bytes + S->used);
             const bool is_small = !plain && q->r.kind == MFFT_REQ_FFT1D && q->r.n <= 4096 &&
                                   (int)small.size() < S->d.coalesce_max && q->r.direction == MFFT_FORWARD;
             if (is_small) {
@@ -487,6 +491,8 @@ void scheduler(mfft_server_s *S)
                 for (Req *q : small) complete(S, q, rc ? rc : MFFT_ERR_HIP);
             }
         }
+        S->busy_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t_busy)
+                          .count();
         if (plain && !S->inflight.empty()) {
             (void)hipStreamSynchronize(S->streams[0]);  // one request at a time
         } else if (!S->inflight.empty() && S->pending.empty()) {
@@ -629,6 +635,7 @@ int mfft_server_get_stats(mfft_server S, mfft_server_stats *st)
     st->static_bytes = S->static_bytes;
     st->inflight_bytes = S->used;
     st->peak_bytes = S->peak;
+    st->busy_us = S->busy_ns / 1000;
     return MFFT_OK;
 }
 

@@ -14,7 +14,9 @@ fft_2d_c2c (tensorcore_fft_conv2d.cpp:79-146). This module
 
 Device buffers come from torch (allocation only); all transforms run in the HIP library.
 """
+import collections
 import ctypes
+import gc
 import time
 
 import numpy as np
@@ -29,6 +31,10 @@ N_FILTERS = 32
 
 class ServerError(RuntimeError):
     pass
+
+
+# One finished request (mfft_completion copied out of the poll buffer).
+Completion = collections.namedtuple("Completion", "tag status submit_us start_us end_us out_scale")
 
 
 def _check(lib, rc, what):
@@ -69,6 +75,7 @@ class Server:
         self.plain = plain
         self.precision = "fp32" if plain else precision
         self.handle = ctypes.c_void_p()
+        self._poll_buf = None
         _check(self.lib, self.lib.mfft_server_create(ctypes.byref(self.handle), ctypes.byref(d)), "create")
 
     # -- data format -------------------------------------------------------------------
@@ -91,10 +98,16 @@ class Server:
         _check(self.lib, self.lib.mfft_server_submit(self.handle, ctypes.byref(r)), "submit")
 
     def poll(self, max_items=4096):
-        buf = (_lib.Completion * max_items)()
+        """Up to max_items finished requests as Completion tuples. The ctypes buffer is
+        allocated once per size and reused; the records are copied out of it, so a caller
+        that keeps them holds no buffer."""
+        buf = self._poll_buf
+        if buf is None or len(buf) < max_items:
+            buf = self._poll_buf = (_lib.Completion * max_items)()
         n = ctypes.c_int(0)
         _check(self.lib, self.lib.mfft_server_poll(self.handle, buf, max_items, ctypes.byref(n)), "poll")
-        return [buf[i] for i in range(n.value)]
+        return [Completion(c.tag, c.status, c.submit_us, c.start_us, c.end_us, c.out_scale)
+                for c in buf[:n.value]]
 
     def wait_idle(self, timeout_s=120.0):
         _check(self.lib, self.lib.mfft_server_wait_idle(self.handle, float(timeout_s)), "wait_idle")
@@ -216,28 +229,42 @@ class Arena:
 
 def replay(server, trace, arena, time_scale=1.0, timeout_s=600.0):
     """Submit the trace at its arrival times (divided by time_scale), poll completions, and
-    return per-request latency (ms) and the throughput over the run."""
+    return per-request latency (ms) and the throughput over the run.
+
+    Python's cyclic garbage collector is paused for the replay: a full collection over a
+    large heap stops this loop for milliseconds, and the arrivals of that gap would then be
+    submitted as one burst the trace does not contain. submit_lag_* report how late the
+    submissions were against the trace's arrival times."""
     n = len(trace)
     done = {}
-    t0 = time.perf_counter()
-    i = 0
-    while i < n or len(done) < n:
-        now = time.perf_counter() - t0
-        while i < n and trace[i][0] / time_scale <= now:
-            r = trace[i]
-            ip, op = arena.pointers(r)
-            server.submit(r[1], ip, op, tag=i, n=r[2], nx=r[3], ny=r[4], batch=r[5], filter_id=r[6])
-            i += 1
-        for c in server.poll():
-            done[c.tag] = c
-        if now > timeout_s:
-            raise ServerError(f"trace replay timed out with {n - len(done)} requests outstanding")
-        if i < n:
-            nxt = trace[i][0] / time_scale - (time.perf_counter() - t0)
-            if nxt > 2e-4:
-                time.sleep(min(nxt, 1e-3) - 1e-4)
-        else:
-            time.sleep(2e-4)
+    lag = np.zeros(n)
+    gc_on = gc.isenabled()
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        i = 0
+        while i < n or len(done) < n:
+            now = time.perf_counter() - t0
+            while i < n and trace[i][0] / time_scale <= now:
+                r = trace[i]
+                ip, op = arena.pointers(r)
+                server.submit(r[1], ip, op, tag=i, n=r[2], nx=r[3], ny=r[4], batch=r[5], filter_id=r[6])
+                lag[i] = time.perf_counter() - t0 - r[0] / time_scale
+                i += 1
+            for c in server.poll():
+                done[c.tag] = c
+            if now > timeout_s:
+                raise ServerError(f"trace replay timed out with {n - len(done)} requests outstanding")
+            if i < n:
+                nxt = trace[i][0] / time_scale - (time.perf_counter() - t0)
+                if nxt > 2e-4:
+                    time.sleep(min(nxt, 1e-3) - 1e-4)
+            else:
+                time.sleep(2e-4)
+    finally:
+        if gc_on:
+            gc.enable()
     bad = [c for c in done.values() if c.status != 0]
     if bad:
         raise ServerError(f"{len(bad)} requests failed (first: tag {bad[0].tag} status {bad[0].status})")
@@ -253,6 +280,8 @@ def replay(server, trace, arena, time_scale=1.0, timeout_s=600.0):
         "span_s": span,
         "req_per_s": n / span,
         "gflops": sum(flops(r) for r in trace) / span / 1e9,
+        "submit_lag_p99_ms": float(np.percentile(lag, 99)) * 1e3,
+        "submit_lag_max_ms": float(lag.max()) * 1e3,
     }
 
 

@@ -334,6 +334,8 @@ typedef struct mfft_server_stats {
     int64_t static_bytes, inflight_bytes;
     int64_t peak_bytes;    /* largest static + in-flight footprint at any admission */
     int64_t paired;        /* conv requests that shared a transform with another */
+    int64_t busy_us;       /* scheduler thread time spent retiring, admitting and launching
+                              (waits and sleeps excluded): busy_us / elapsed is its load */
 } mfft_server_stats;
 
 typedef struct mfft_server_s *mfft_server;
