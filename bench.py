@@ -499,6 +499,34 @@ def serve_section(seconds, sat_requests=2000, p99_bound_ms=2.0):
     return out
 
 
+# ------------------------------------------------------------------------- MFMA use
+MFMA_FLOPS = 16 * 16 * 32 * 2  # one v_mfma_f32_16x16x32_f16
+MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense fp16
+
+
+def mfma_utilisation(passes, pass_ms):
+    """Matrix-core use of the split-fp16 passes. Every radix-16 stage is a complex DFT16
+    on 16 lines at a time: two v_mfma_f32_16x16x32_f16 per real product (the complex
+    product is folded into K = 32, the accumulator holds 128 of the 256 complex outputs),
+    three products (Fl*Xh + Fh*Xl + Fh*Xh), two for a first stage that reads exact fp16
+    input (no Xl). The count per launch agrees with rocprofv3's SQ_INSTS_MFMA
+    (profiles/fft_fast_split_n65536x256_mfma.md)."""
+    out = []
+    for p, ms in zip(passes, pass_ms):
+        if p["generic"] or not p["kernel"].startswith("fast"):
+            continue
+        n16 = sum(1 for r in p["radix"] if r == 16)
+        groups = p["points"] // 256
+        exact_first = p["in_elem_bytes"] == 4  # c16 in (the split path's fp16 buffers)
+        per_group = 2 * (2 if exact_first else 3) + 2 * 3 * (n16 - 1) if n16 else 0
+        inst = groups * per_group
+        tf = inst * MFMA_FLOPS / (ms * 1e-3) / 1e12
+        out.append({"kernel": p["kernel"], "R": p["R"], "instructions_per_launch": inst,
+                    "avg_launch_ms": round(ms, 5), "achieved_tflops": round(tf, 1),
+                    "peak_tflops": MFMA_PEAK_TFLOPS, "frac": round(tf / MFMA_PEAK_TFLOPS, 4)})
+    return out
+
+
 # ------------------------------------------------------------------------- conv2d
 def conv_section(steps, warmup):
     """FFT 'same' convolution: 2048x2048 fp32 images U[0,1], batch 16, one 31x31 filter
@@ -682,6 +710,7 @@ def main():
     }
     roofline["copy_gbs"] = copy_bandwidth()
     roofline["frac_of_copy"] = round(achieved / roofline["copy_gbs"], 4)
+    mfma = mfma_utilisation(head["passes"], pp)
     n = math.prod(shape)
     whole_bytes = batch * n * 8  # fp16 complex in + out
     line = {
@@ -714,6 +743,7 @@ def main():
         "comparators": comps,
         "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3) if "fp32" in comps else None,
         "roofline": roofline,
+        "mfma": mfma,
     }
     if len(shape) == 2:
         rl2, rmx = round_trip(shape, batch, xh, dev, sample)
