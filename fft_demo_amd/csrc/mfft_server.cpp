@@ -58,6 +58,7 @@ struct Req {
     double t_submit = 0, t_start = 0;
     long long need = 0;
     int stream = -1;
+    long long work = 0;       // counted in its stream's queued work while in flight
     hipEvent_t ev = nullptr;  // shared by the requests of one coalesced launch
     bool owns_ev = false;
     double out_scale = 1.0;
@@ -79,6 +80,7 @@ struct mfft_server_s {
     std::vector<hipStream_t> streams;
     std::vector<void *> slabs;
     std::vector<int> outstanding;  // requests in flight per stream
+    std::vector<long long> queued_work;  // complex points of the requests in flight per stream
     std::vector<hipEvent_t> free_events;  // completion events for reuse
     long long slab_bytes = 0;
     std::atomic<long long> static_bytes{0};  // slabs + conv operators + spectra
@@ -143,13 +145,21 @@ mfft_plan get_plan(mfft_server_s *S, int kind, int nx, int ny, int dir)
 
 // Streams form two lanes: requests below kBulkPoints complex points go to the fast lane,
 // larger ones to the bulk lane, so a small request never queues behind a large one on
-// its stream. Within a lane, the stream with the fewest requests in flight.
+// its stream. Within a lane, the stream with the least work in flight (complex points of
+// its requests; a conv request counts its image twice, for the forward and the inverse
+// transform), so a request does not wait behind a large one while the lane's other
+// stream has little queued.
 constexpr long long kBulkPoints = 1LL << 20;
 
 long long req_points(const mfft_request &r)
 {
     if (r.kind == MFFT_REQ_FFT1D) return (long long)r.n * r.batch;
     return (long long)r.nx * r.ny * (r.kind == MFFT_REQ_FFT2D ? r.batch : 1);
+}
+
+long long req_work(const mfft_request &r)
+{
+    return req_points(r) * (r.kind == MFFT_REQ_CONV2D ? 2 : 1);
 }
 
 // Filters whose sizes have the same parity share a frame (the class's largest height and
@@ -268,8 +278,19 @@ int pick_stream(mfft_server_s *S, bool bulk)
     }
     int best = lo;
     for (int i = lo + 1; i < hi; ++i)
-        if (S->outstanding[i] < S->outstanding[best]) best = i;
+        if (S->queued_work[i] < S->queued_work[best] ||
+            (S->queued_work[i] == S->queued_work[best] && S->outstanding[i] < S->outstanding[best]))
+            best = i;
     return best;
+}
+
+// q runs on stream st until it completes
+void track(mfft_server_s *S, Req *q, int st)
+{
+    q->stream = st;
+    q->work = req_work(q->r);
+    S->outstanding[st]++;
+    S->queued_work[st] += q->work;
 }
 
 hipEvent_t take_event(mfft_server_s *S)
@@ -311,8 +332,7 @@ int launch_one(mfft_server_s *S, Req *q, int st)
     if (!q->ev) return MFFT_ERR_HIP;
     q->owns_ev = true;
     if (hipEventRecord(q->ev, s) != hipSuccess) return MFFT_ERR_HIP;
-    q->stream = st;
-    S->outstanding[st]++;
+    track(S, q, st);
     S->launches++;
     return MFFT_OK;
 }
@@ -332,8 +352,8 @@ int launch_conv_two(mfft_server_s *S, Req *a, Req *b, ConvSlot *c, int ia, int i
     a->ev = b->ev = ev;
     a->owns_ev = true;
     b->owns_ev = false;
-    a->stream = b->stream = c->stream;
-    S->outstanding[c->stream] += 2;
+    track(S, a, c->stream);
+    track(S, b, c->stream);
     S->launches++;
     S->paired += 2;
     return MFFT_OK;
@@ -354,7 +374,10 @@ void complete(mfft_server_s *S, Req *q, int status)
         S->completed++;
     }
     if (q->admitted) S->used -= q->need;
-    if (q->stream >= 0) S->outstanding[q->stream]--;
+    if (q->stream >= 0) {
+        S->outstanding[q->stream]--;
+        S->queued_work[q->stream] -= q->work;
+    }
     if (q->owns_ev && q->ev) S->free_events.push_back(q->ev);
     delete q;
 }
@@ -482,8 +505,7 @@ void scheduler(mfft_server_s *S)
                 for (size_t k = 0; k < small.size(); ++k) {
                     small[k]->ev = ev;
                     small[k]->owns_ev = k == 0;
-                    small[k]->stream = st;
-                    S->outstanding[st]++;
+                    track(S, small[k], st);
                     S->inflight.push_back(small[k]);
                 }
             } else {
@@ -542,6 +564,7 @@ int mfft_server_create(mfft_server *out, const mfft_server_desc *d)
     S->streams.resize(S->d.streams);
     S->slabs.resize(S->d.streams, nullptr);
     S->outstanding.assign(S->d.streams, 0);
+    S->queued_work.assign(S->d.streams, 0);
     for (int i = 0; i < S->d.streams; ++i) {
         if (hipStreamCreateWithFlags(&S->streams[i], hipStreamNonBlocking) != hipSuccess ||
             hipMalloc(&S->slabs[i], S->slab_bytes) != hipSuccess) {
