@@ -99,8 +99,9 @@ def time_plan(plan, x, out, steps, warmup, stream):
     alone (mfft_exec_pass), each between a start and a stop HIP event that the runtime
     records around the kernel itself on the exec stream (its execution window, as a kernel
     trace reports it); the average is the pass's launch time. Plans that run their batch in
-    several chunks fall back to two events around `steps` back-to-back launches of the pass
-    (launch gaps included). Returns (total_ms, [per-pass avg ms])."""
+    several chunks fall back to two events around `steps` back-to-back runs of the pass
+    (one launch per chunk, launch gaps included), divided by the number of launches.
+    Returns (total_ms, [per-pass avg ms per launch])."""
     import torch
 
     from fft_demo_amd.hiptime import EventPool
@@ -144,8 +145,10 @@ def time_plan(plan, x, out, steps, warmup, stream):
     global PASS_TIMING
     per_pass = []
     single = plan.info["batch_chunk"] >= plan.batch
+    nchunks = -(-plan.batch // plan.info["batch_chunk"])
     PASS_TIMING = ("kernel window: start/stop HIP events recorded around each launch (hipExtLaunchKernel)"
-                   if single else "two HIP events around back-to-back launches of the pass")
+                   if single else f"two HIP events around back-to-back runs of the pass ({nchunks} chunk launches "
+                                  "each), per launch")
     pool = EventPool(2 * steps)
     for k in range(npass):
         if single:
@@ -164,7 +167,7 @@ def time_plan(plan, x, out, steps, warmup, stream):
                     raise RuntimeError(lib.mfft_last_error().decode())
             pool.record(1, stream.cuda_stream)
             torch.cuda.synchronize()
-            per_pass.append(pool.elapsed_ms(0, 1) / steps)
+            per_pass.append(pool.elapsed_ms(0, 1) / steps / nchunks)
     pool.close()
     # leave `out` holding a full exec's result
     plan.exec(x, out, stream=stream)

@@ -1,5 +1,5 @@
 // Fast tile kernels of the matrix-core FFT (power-of-two line lengths 64..4096; the
-// strided kind 64..256).
+// strided kind 64..1024).
 //
 // Same method as the generic tile kernel (radix-16 DFT-matrix products on MFMA, Stockham
 // schedule, LDS between stages) with the HBM traffic moved into the stages themselves:
@@ -12,15 +12,22 @@
 //   KIND_ROWS     points contiguous (in_ps = out_ps = 1); lanes walk the points of a line.
 //   KIND_ROWS_TR  points contiguous on input, lines contiguous on output (out_s0 = 1):
 //                 the last stage switches the lanes to walk the lines.
-//   KIND_COLS     lines contiguous on both sides (in_s0 = out_s0 = 1, T = 16); lanes walk
-//                 the 16 lines of the tile (strided passes of the four-step method).
+//   KIND_COLS     lines contiguous on both sides (in_s0 = out_s0 = 1, T = 32 up to 256
+//                 points, 16 at 512, 8 at 1024); lanes walk the lines of the tile
+//                 (strided passes: the first pass of the four-step method, the columns
+//                 of a 2D transform).
 #pragma once
 #include "mfft_common.h"
 
 namespace mfft {
 
 enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2 };
-constexpr int kColsT = 32;  // lines per tile of the strided kind
+// Lines per tile of the strided kind: 32 (a full 128-byte segment of fp16 pairs per point)
+// up to 256 points, then R * T = 8192 points: a 16-wave workgroup keeps two 16-column MFMA
+// groups per wave, whose accumulators, stage values and next-tile operands fit the 128
+// registers a wave has at 4 waves per SIMD (four groups per wave spill to scratch).
+constexpr int kColsMaxR = 1024;
+constexpr int cols_lines(int R) { return R <= 256 ? 32 : 8192 / R; }
 
 typedef void (*fast_fn)(TileArgs);
 fast_fn find_fast_kernel(int prec, int R, int kind, int in_fmt, int out_fmt);

@@ -13,7 +13,7 @@ fast_fn find_fast_kernel(int prec, int R, int kind, int inf, int outf)
     return nullptr;
 }
 
-int fast_tile_lines(int R, int kind) { return kind == KIND_COLS ? kColsT : kMaxTile / R; }
+int fast_tile_lines(int R, int kind) { return kind == KIND_COLS ? cols_lines(R) : kMaxTile / R; }
 int fast_threads(int prec, int R, int kind)
 {
     // the host side of fast_waves<PREC, KIND, R>()

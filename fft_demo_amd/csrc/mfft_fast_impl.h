@@ -48,13 +48,13 @@ struct Elem<FMT_C32> {
 };
 
 // ------------------------------------------------------------------ geometry
-// Waves per workgroup. A tile is T lines of R points (T = kMaxTile / R; kColsT for the
-// strided kind). fp32: one 16-point column per thread in the radix-16 stages. Matrix-core
+// Waves per workgroup. A tile is T lines of R points (T = kMaxTile / R; cols_lines(R) for
+// the strided kind). fp32: one 16-point column per thread in the radix-16 stages. Matrix-core
 // precisions: two 16-column MFMA groups per wave, 4..16 waves.
 template <int PREC, int KIND, int R>
 constexpr int fast_waves()
 {
-    constexpr int T = KIND == KIND_COLS ? kColsT : kMaxTile / R;
+    constexpr int T = KIND == KIND_COLS ? cols_lines(R) : kMaxTile / R;
     constexpr int groups = R * T / 256;
     return PREC == PREC_FP32 ? R * T / 16 / 64 : (groups / 2 < 4 ? 4 : (groups / 2 > 16 ? 16 : groups / 2));
 }
@@ -64,7 +64,7 @@ struct Geo {
     static constexpr int NW = NW_;            // waves per workgroup
     static constexpr int NT = NW_ * 64;       // threads per workgroup
     static constexpr int R16 = R / 16;
-    static constexpr int T = KIND == KIND_COLS ? kColsT : (kMaxTile / R);
+    static constexpr int T = KIND == KIND_COLS ? cols_lines(R) : (kMaxTile / R);
     // ROWS*: [line][point], padded points and an odd line pitch (lanes walk either the
     // points of a line or, in the transposed last stage, the lines);
     // COLS: [point][T lines] (every access walks consecutive lines of one point).
@@ -826,10 +826,13 @@ fast_fn pick_r(int R)
     case 256: return fft_fast<PREC, 256, KIND, INF, OUTF>;
     default: break;
     }
+    switch (R) {
+    case 512: return fft_fast<PREC, 512, KIND, INF, OUTF>;
+    case 1024: return fft_fast<PREC, 1024, KIND, INF, OUTF>;
+    default: break;
+    }
     if constexpr (KIND != KIND_COLS) {
         switch (R) {
-        case 512: return fft_fast<PREC, 512, KIND, INF, OUTF>;
-        case 1024: return fft_fast<PREC, 1024, KIND, INF, OUTF>;
         case 2048: return fft_fast<PREC, 2048, KIND, INF, OUTF>;
         case 4096: return fft_fast<PREC, 4096, KIND, INF, OUTF>;
         default: break;

@@ -171,7 +171,10 @@ struct mfft_plan_s {
 static int fmt_bytes(int fmt) { return fmt == FMT_C32 ? 8 : 4; }
 
 // ------------------------------------------------------------------ axis planning
-static bool single_pass_ok(int R, long long ps_in, const std::vector<Level> &lines)
+// fast_cols: the pass may run on the strided matrix-core kernel (natural-order input), which
+// holds 16 or 32 lines of up to kColsMaxR points per tile; the tile kernels hold at most
+// kMaxTile points, so a strided single pass there needs R <= kMaxTile / 16 for 16 lines.
+static bool single_pass_ok(int R, long long ps_in, const std::vector<Level> &lines, bool fast_cols)
 {
     if (R < 2 || R > kMaxTile) return false;
     std::vector<int> rad;
@@ -180,7 +183,8 @@ static bool single_pass_ok(int R, long long ps_in, const std::vector<Level> &lin
     bool has_unit = false;
     for (auto &l : lines)
         if (l.in_stride == 1) has_unit = true;
-    return has_unit ? R * 16 <= kMaxTile : true;
+    if (!has_unit) return true;
+    return R * 16 <= kMaxTile || (fast_cols && is_pow2(R) && R >= 64 && R <= kColsMaxR);
 }
 
 static bool factorable(long long n)
@@ -217,7 +221,8 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     const long long odd = odd_part(L);
     const bool odd_split = perm_axis < 0 && split_hint != 1 && odd > 1 && small_odd(odd) && L / odd >= 16 &&
                            L / odd <= kMaxTile;
-    if (!odd_split && single_pass_ok((int)std::min<long long>(L, 1 << 30), ps_in, lines)) {
+    const bool fast_cols = perm_axis < 0 && split_hint <= 1;  // no split asked for
+    if (!odd_split && single_pass_ok((int)std::min<long long>(L, 1 << 30), ps_in, lines, fast_cols)) {
         PassSpec p;
         p.R = (int)L;
         p.ps_in = ps_in;
@@ -244,7 +249,8 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     long long L1 = 0;
     if (odd_split) {
         L1 = odd;
-    } else if (split_hint > 1 && L % split_hint == 0 && split_hint <= 256 && factorable(split_hint)) {
+    } else if (split_hint > 1 && L % split_hint == 0 && factorable(split_hint) &&
+               (split_hint <= 256 || (is_pow2(split_hint) && split_hint <= kColsMaxR))) {
         L1 = split_hint;
     } else if (is_pow2(L) && ps_in != 1 && L >= 512 && L <= 4096) {
         // strided axis (2D columns) of 512..4096 points: both passes are strided; the first
@@ -311,7 +317,14 @@ static int finish_passes(mfft_plan_s *p)
         }
         bool strided = ps.ps_in != 1;
         bool pow2 = is_pow2(ps.R) && ps.R >= 16;
-        if (strided && unit_in >= 0 && ps.R * 16 <= kMaxTile) {
+        const int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
+        const int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+        // strided lines longer than a 16-line tile kernel holds: the strided matrix-core
+        // kernel when it exists for the formats (16 or 32 lines of up to kColsMaxR points)
+        const bool cols_long = strided && unit_in >= 0 && ps.R * 16 > kMaxTile && pow2 && ps.R <= kColsMaxR &&
+                               ps.perm_axis < 0 && ps.lines[unit_in].out_stride == 1 &&
+                               find_fast_kernel(p->d.precision, ps.R, KIND_COLS, in_fmt, out_fmt) != nullptr;
+        if (strided && unit_in >= 0 && (ps.R * 16 <= kMaxTile || cols_long)) {
             ps.l0 = unit_in;
             ps.TL = 16;
         } else if (!strided) {
@@ -334,8 +347,6 @@ static int finish_passes(mfft_plan_s *p)
         // short strided DFTs: streaming kernel, one thread per line
         if (ps.perm_axis < 0 && ps.ps_in != 1 &&
             (ps.R == 2 || ps.R == 4 || ps.R == 8 || ps.R == 16 || small_odd(ps.R))) {
-            int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
-            int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
             const void *fn = find_small_kernel(ps.R, in_fmt, out_fmt);
             if (fn) {
                 ps.small = true;
@@ -349,8 +360,6 @@ static int finish_passes(mfft_plan_s *p)
         // fast kernel: power of two 64..4096, no fused permutation
         if (!ps.small && pow2 && ps.R >= 64 && ps.R <= kMaxTile && ps.perm_axis < 0) {
             int kind = -1, l0 = -1;
-            int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
-            int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
             if (ps.ps_in == 1 && ps.ps_out == 1) {
                 kind = KIND_ROWS;
                 l0 = 0;
@@ -359,7 +368,7 @@ static int finish_passes(mfft_plan_s *p)
             } else if (ps.ps_in == 1 && unit_out >= 0) {
                 kind = KIND_ROWS_TR;
                 l0 = unit_out;
-            } else if (ps.ps_in != 1 && ps.R <= 256) {
+            } else if (ps.ps_in != 1 && ps.R <= kColsMaxR) {
                 for (int i = 0; i < (int)ps.lines.size(); ++i)
                     if (ps.lines[i].in_stride == 1 && ps.lines[i].out_stride == 1) { l0 = i; break; }
                 if (l0 >= 0) kind = KIND_COLS;
@@ -369,7 +378,7 @@ static int finish_passes(mfft_plan_s *p)
                 ps.fast_kind = kind;
                 ps.l0 = l0;
                 ps.T = fast_tile_lines(ps.R, kind);
-                ps.TL = kind == KIND_COLS ? kColsT : 1;
+                ps.TL = kind == KIND_COLS ? cols_lines(ps.R) : 1;
                 ps.fn = fn;
                 int dev = 0, ncu = 0, nb = 0;
                 HIP_TRY(hipGetDevice(&dev));

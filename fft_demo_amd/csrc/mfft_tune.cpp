@@ -12,6 +12,7 @@
 // Results live in a process-wide cache, saved to and loaded from a text file, and a plan
 // created with split_hint = MFFT_SPLIT_TUNED takes its split and chunk from the cache.
 #include "../../include/mfft.h"
+#include "mfft_fast.h"
 
 #include <hip/hip_runtime.h>
 
@@ -139,15 +140,15 @@ int mfft_autotune(int nx, int ny, int64_t batch, const mfft_desc *d, int reps, m
     std::vector<int> splits{0};
     if (!two_d && d->in_order == MFFT_ORDER_NATURAL) {
         if (is_pow2(n) && n > 4096)
-            for (int l1 = 2; l1 <= 256; l1 *= 2)
+            for (int l1 = 2; l1 <= mfft::kColsMaxR; l1 *= 2)
                 if (n / l1 >= 16 && n % l1 == 0) splits.push_back(l1);
         if (odd_part(n) > 1) splits.push_back(1);
     }
     if (two_d && odd_part(nx) * odd_part(ny) > 1 && nx <= 4096 && ny <= 4096) splits.push_back(1);
     // 2D: the first-pass length of a split column axis
     if (two_d && d->in_order == MFFT_ORDER_NATURAL && is_pow2(nx) && nx > 256)
-        for (int l1 = 2; l1 <= 256; l1 *= 2)
-            if (nx / l1 >= 16 && nx % l1 == 0) splits.push_back(l1);
+        for (int l1 = 2; l1 <= mfft::kColsMaxR; l1 *= 2)
+            if (nx / l1 >= (l1 > 256 ? 2 : 16) && nx % l1 == 0) splits.push_back(l1);
     // candidate chunks: 16 MiB .. 1 GiB of intermediates in flight between the passes (at
     // least one transform, at most the whole batch), capped by the caller's limit
     const long long per = n * 8;

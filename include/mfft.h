@@ -64,7 +64,8 @@ typedef struct mfft_desc {
     int in_order;             /* MFFT_ORDER_* */
     int64_t workspace_limit;  /* bytes of intermediate buffer per plan (0 = default) */
     int split_hint;           /* first-pass length of a four-step split (1D; the column axis
-                                 of a 2D plan when that axis is split) (0 = planner's choice;
+                                 of a 2D plan, which is otherwise one pass up to 1024 points):
+                                 up to 256, or a power of two up to 1024 (0 = planner's choice;
                                  1 = odd * 2^a lengths in one mixed-radix pass;
                                  MFFT_SPLIT_TUNED = split and workspace_limit from the
                                  autotuner's cache, planner's choice when absent) */

@@ -167,3 +167,17 @@ def test_three_pass_lengths_narrow_intermediates(dev, n, prec, mid):
     print(f"n=2^{n.bit_length() - 1} {prec} mid={mid}: rel_l2={l2:.2e} max={mx:.2e} passes={kinds}")
     assert len(kinds) == 3
     assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp16", "fp32"])
+@pytest.mark.parametrize("n,l1", [(1 << 19, 512), (1 << 20, 1024), (1 << 16, 1024)])
+def test_four_step_long_strided_first_pass(dev, n, l1, prec):
+    """Four-step splits whose strided first pass has 512 or 1024 points (the strided
+    matrix-core kernel at 16 and 8 lines per tile, four-step twiddle fused in its store)."""
+    y, ref, plan = _run(dev, n, 2, prec, seed=l1, split_hint=l1)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    assert kinds[0] == ("fast_cols", l1), kinds
+    l2, mx = rel_errors(y, ref)
+    print(f"N={n} split {l1} {prec}: rel_l2={l2:.3e} rel_max={mx:.3e} passes={kinds}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]

@@ -81,3 +81,36 @@ def test_2d_short_axes_other_precisions(dev, prec, tol):
         l2, _ = rel_errors(y, oracle.fft2d(x))
         print(f"2D {nx}x{ny} {prec}: rel_l2={l2:.2e} kernels={[p['kernel'] for p in plan.info['passes']]}")
         assert l2 <= tol, (nx, ny, prec, l2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("split_fp16", 1e-3), ("fp16", 4e-3), ("bf16", 3e-2), ("fp32", 1e-5)])
+def test_2d_long_columns_one_strided_pass(dev, prec, tol):
+    """Column axes of 512 and 1024 points in ONE pass of the strided matrix-core kernel
+    (16 and 8 lines per tile), full and ragged line counts (ny not a multiple of the tile's
+    lines), forward and inverse."""
+    fmt = {"split_fp16": "c16", "fp16": "c16", "bf16": "cbf16", "fp32": "c32"}[prec]
+    for nx, ny, b, sign in [(1024, 40, 2, -1), (1024, 36, 1, 1), (512, 24, 2, -1), (512, 20, 1, 1),
+                            (1024, 1024, 1, -1)]:
+        plan = _plan(nx, ny, b, precision=prec, direction=sign)
+        kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+        assert kinds[-1] == ("fast_cols", nx), kinds
+        x = complex_input((b, nx, ny), nx + 7 * ny, fmt)
+        y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
+        l2, mx = rel_errors(y, oracle.fft2d(x, sign=sign))
+        print(f"2D {nx}x{ny} b={b} {prec} sign={sign}: rel_l2={l2:.2e} max={mx:.2e} passes={kinds}")
+        assert l2 <= tol, (nx, ny, prec, l2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mid", [None, "c16"])
+def test_2d_long_columns_split_hint_keeps_split(dev, mid):
+    """split_hint on a 1024-point column axis still splits it (64 on the strided kernel,
+    16 streamed): the tuner's other candidate."""
+    plan = _plan(1024, 64, 1, precision="split_fp16", split_hint=64, mid_fmt=mid)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    assert ("fast_cols", 64) in kinds and len(kinds) == 3, kinds
+    x = complex_input((1, 1024, 64), 5)
+    y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
+    l2, mx = rel_errors(y, oracle.fft2d(x))
+    assert l2 <= 1e-3 and mx <= 4e-3, (l2, mx)
