@@ -267,11 +267,20 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
         int m1 = (m + 1) / 2;
         if (m1 > 8) m1 = 8;
         if (m - m1 > 12 && m1 < 8) m1 = std::min(8, m - 12);
-        // 2^19..2^21 with 16-bit intermediates: a second pass of 2048 or 4096 points would
-        // hold one or two lines per tile and store them transposed in 4- or 8-byte pieces;
-        // three passes instead, the rest (8192 = 128 x 64) split again below (split fp16:
-        // 1.05-1.3x faster; with fp32 intermediates the third pass costs more than it saves)
-        if (narrow && m >= 19 && m <= 21) m1 = m - 13;
+        // 2^17..2^19 (contiguous lines): a long strided first pass -- 1024 points at 8 lines
+        // per tile with 16-bit intermediates, 512 at 16 lines with 32-bit ones -- and the
+        // rest (128..1024 points) in one pass; the autotuner's pick on MI355X for split fp16
+        // and fp32, 1.02-1.18x the 256-point first pass (and the three passes below)
+        if (ps_in == 1 && m >= 17 && m <= 19) {
+            m1 = narrow ? 10 : 9;
+        } else if (narrow && m >= 19 && m <= 21) {
+            // 2^20..2^21 with 16-bit intermediates: a second pass of 2048 or 4096 points
+            // would hold one or two lines per tile and store them transposed in 4- or
+            // 8-byte pieces; three passes instead, the rest (8192 = 128 x 64) split again
+            // below (split fp16: 1.05-1.3x faster; with fp32 intermediates the third pass
+            // costs more than it saves)
+            m1 = m - 13;
+        }
         L1 = 1LL << m1;
     } else {
         for (long long c = 256; c >= 2; --c)

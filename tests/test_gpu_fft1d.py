@@ -159,13 +159,31 @@ def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid, timed):
 @pytest.mark.parametrize("n", [1 << 19, 1 << 20, 1 << 21])
 @pytest.mark.parametrize("prec,mid", [("split_fp16", "c16"), ("fp16", None)])
 def test_three_pass_lengths_narrow_intermediates(dev, n, prec, mid):
-    """2^19..2^21 with 16-bit intermediates plan three passes (first pass 2^(m-13), then
-    8192 = 128 x 64); tolerance as for the precision."""
+    """With 16-bit intermediates 2^20 and 2^21 plan three passes (first pass 2^(m-13), then
+    8192 = 128 x 64) and 2^19 two (a 1024-point strided first pass, then 512 points);
+    tolerance as for the precision."""
     y, ref, plan = _run(dev, n, 2, prec, mid_fmt=mid)
     l2, mx = rel_errors(y, ref)
     kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
     print(f"n=2^{n.bit_length() - 1} {prec} mid={mid}: rel_l2={l2:.2e} max={mx:.2e} passes={kinds}")
-    assert len(kinds) == 3
+    if n == 1 << 19:
+        assert kinds == [("fast_cols", 1024), ("fast_rows_tr", 512)], kinds
+    else:
+        assert len(kinds) == 3
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,mid,l1", [("split_fp16", "c16", 1024), ("split_fp16", None, 512),
+                                         ("fp16", None, 1024), ("fp32", None, 512)])
+@pytest.mark.parametrize("n", [1 << 17, 1 << 18])
+def test_default_long_first_pass(dev, n, prec, mid, l1):
+    """2^17..2^19 plan a long strided first pass by default: 1024 points with 16-bit
+    intermediates, 512 with 32-bit ones; the rest in one contiguous pass."""
+    y, ref, plan = _run(dev, n, 3, prec, seed=n % 997, mid_fmt=mid)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    assert kinds == [("fast_cols", l1), ("fast_rows_tr", n // l1)], kinds
+    l2, mx = rel_errors(y, ref)
     assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
 
 
