@@ -251,7 +251,11 @@ def rocfft_point(shape, batch, xh, steps, warmup, dev):
 
 def cpu_baseline(shape, batch, xh):
     """FP64 oracle FFT on the host: the reference's single-thread batch-loop harness shape
-    (main.cpp:68-80) on 1 core, plus the OpenMP-over-batch variant on all cores."""
+    (main.cpp:68-80) on 1 core, plus the OpenMP-over-batch variant on all cores; one
+    warm-up run, then the median of five. The reference's own CPU path calls FFTW
+    (FFTW_ESTIMATE); libfftw3 is looked up and its absence stated."""
+    import ctypes.util
+
     import numpy as np
 
     import oracle
@@ -260,7 +264,7 @@ def cpu_baseline(shape, batch, xh):
     sample = batch if len(shape) == 1 else max(1, batch // 16)
     data = np.ascontiguousarray(xh[:sample].reshape(-1))
     runs = []
-    for _ in range(3):
+    for _ in range(6):
         t = time.perf_counter()
         if len(shape) == 1:
             oracle.fft_batch_loop(data, n, sample)
@@ -268,28 +272,32 @@ def cpu_baseline(shape, batch, xh):
             xc = xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1]
             oracle.fft2d(xc, threads=1)
         runs.append(time.perf_counter() - t)
+    runs = runs[1:]
     t1 = sorted(runs)[len(runs) // 2]
     ncores = os.cpu_count() or 1
     xc = xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1]
     runs = []
-    for _ in range(3):
+    for _ in range(6):
         t = time.perf_counter()
         if len(shape) == 1:
             oracle.fft(xc, threads=0)
         else:
             oracle.fft2d(xc, threads=0)
         runs.append(time.perf_counter() - t)
+    runs = runs[1:]
     ta = sorted(runs)[len(runs) // 2]
     f = flops(shape, sample)
+    fftw = ctypes.util.find_library("fftw3")
     return {
         "value": f / t1 / 1e9,
         "unit": "GFLOP/s",
         "cores": 1,
         "kind": "port",
         "sample": f"{sample} transforms of the workload, FP64 Stockham oracle, single-thread batch loop "
-                  f"(reference harness shape main.cpp:68-80), median of 3",
+                  f"(reference harness shape main.cpp:68-80), one warm-up then median of 5",
         "all_cores": {"value": f / ta / 1e9, "unit": "GFLOP/s", "cores": ncores},
         "host_cpu": _cpu_model(),
+        "fftw": f"libfftw3 found ({fftw}), not timed" if fftw else "FFTW unavailable (no libfftw3 on the host)",
     }
 
 
