@@ -17,6 +17,7 @@
 #include "../../include/mfft.h"
 
 #include <hip/hip_runtime.h>
+#include <sys/prctl.h>
 
 #include <algorithm>
 #include <atomic>
@@ -385,6 +386,10 @@ void complete(mfft_server_s *S, Req *q, int status)
 void scheduler(mfft_server_s *S)
 {
     (void)hipSetDevice(S->device);
+    // Completions are found by polling events every few microseconds while requests are
+    // in flight; the default 50 us timer slack of a Linux thread would stretch each poll
+    // period, and every request's latency with it, to about 55 us.
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     const bool plain = S->d.plain != 0;
     std::vector<Req *> small;
     std::vector<mfft_batch_item> items;
@@ -518,7 +523,9 @@ void scheduler(mfft_server_s *S)
         if (plain && !S->inflight.empty()) {
             (void)hipStreamSynchronize(S->streams[0]);  // one request at a time
         } else if (!S->inflight.empty() && S->pending.empty()) {
-            std::this_thread::sleep_for(std::chrono::microseconds(5));
+            // until the next poll of the in-flight events, or a submission (which notifies)
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->incoming.empty()) S->cv.wait_for(lk, std::chrono::microseconds(5));
         }
     }
 }
