@@ -695,9 +695,26 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
 // per case so no variant carries another's registers, selects or branches (a masked and
 // an unmasked load path in one loop made the compiler wait for every outstanding memory
 // operation, the previous tile's stores included, before each tile's loads).
+// Stage twiddles and the (small) four-step tables into LDS, the DFT16 operands into
+// registers.
+template <int PREC, int R, int KIND>
+__device__ __forceinline__ void stage_tables(const TileArgs &a, float2 *s_stw, float2 *s_tw, Ops<PREC> &op)
+{
+    typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
+    constexpr int NS = __builtin_ctz(R) / 4;
+    for (int i = threadIdx.x; i < R; i += G::NT) s_stw[i] = a.stage_tw_s[i];
+    if (a.tw_N && a.tw_lds) {
+        const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
+        // the tail-as-last stage (NS == 1) is VALU: it takes the unscaled lo table
+        const float2 *lo = NS == 1 ? a.tw_lo : a.tw_lo_s;
+        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = lo[i];
+        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[G::TWC + i] = a.tw_hi[i];
+    }
+    if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
+}
+
 template <int PREC, int R, int KIND, int INF, int OUTF, bool TW, bool FULL>
-__device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const float2 *stw, const float2 *s_tw,
-                                           const Ops<PREC> &op)
+__device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float2 *stw, float2 *s_tw)
 {
     typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
     constexpr int L2 = __builtin_ctz(R);
@@ -725,6 +742,11 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, const
     walk.init(a, G::T, tile < end ? tile : 0, step);
     Origin o = walk.origin(a, G::T);
     if (tile < end) cur.template load<FULL>(a, o);
+    // the tables are loaded after the first tile's operands, so the two global-memory
+    // round trips overlap (a workgroup with one tile waits for one, not two)
+    Ops<PREC> op;
+    stage_tables<PREC, R, KIND>(a, stw, s_tw, op);
+    __syncthreads();
     // the loop starts with no loads in flight: otherwise the first tile's pending loads
     // make every iteration wait for all loads (its own prefetch included) before its
     // first product
@@ -786,34 +808,17 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
     constexpr int TAIL = 1 << (L2 % 4);
     static_assert(NS >= 2 || (NS == 1 && TAIL > 1), "fast kernels need R >= 32");
 
-    // tables: stage twiddles and (small) four-step tables into LDS
-    const float2 *stw = a.stage_tw_s;
-    if constexpr (G::STW > 0) {
-        for (int i = threadIdx.x; i < R; i += G::NT) s_stw[i] = a.stage_tw_s[i];
-        stw = s_stw;
-    }
-    if (a.tw_N && a.tw_lds) {
-        const int nlo = 1 << a.tw_bits, nhi = (a.tw_N + nlo - 1) >> a.tw_bits;
-        // the tail-as-last stage (NS == 1) is VALU: it takes the unscaled lo table
-        const float2 *lo = NS == 1 ? a.tw_lo : a.tw_lo_s;
-        for (int i = threadIdx.x; i < nlo; i += G::NT) s_tw[i] = lo[i];
-        for (int i = threadIdx.x; i < nhi; i += G::NT) s_tw[G::TWC + i] = a.tw_hi[i];
-    }
-    Ops<PREC> op;
-    if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
-    __syncthreads();
-
     const bool full = a.c0 % G::T == 0;
     if (a.tw_N) {
         if (full)
-            fast_tiles<PREC, R, KIND, INF, OUTF, true, true>(a, lds, stw, s_tw, op);
+            fast_tiles<PREC, R, KIND, INF, OUTF, true, true>(a, lds, s_stw, s_tw);
         else
-            fast_tiles<PREC, R, KIND, INF, OUTF, true, false>(a, lds, stw, s_tw, op);
+            fast_tiles<PREC, R, KIND, INF, OUTF, true, false>(a, lds, s_stw, s_tw);
     } else {
         if (full)
-            fast_tiles<PREC, R, KIND, INF, OUTF, false, true>(a, lds, stw, s_tw, op);
+            fast_tiles<PREC, R, KIND, INF, OUTF, false, true>(a, lds, s_stw, s_tw);
         else
-            fast_tiles<PREC, R, KIND, INF, OUTF, false, false>(a, lds, stw, s_tw, op);
+            fast_tiles<PREC, R, KIND, INF, OUTF, false, false>(a, lds, s_stw, s_tw);
     }
 }
 
