@@ -95,10 +95,11 @@ def time_plan(plan, x, out, steps, warmup, stream):
     """Time `steps` back-to-back execs between two HIP events on the exec stream; when an
     exec is shorter than 0.5 ms (the Python wrapper's launch overhead would then leave gaps
     in the queue) a hipGraph holding the `steps` execs is timed too and the faster is reported.
-    Each pass is then timed on its own (for the roofline): `steps` launches of that pass
-    alone (mfft_exec_pass), each between a start and a stop HIP event that the runtime
-    records around the kernel itself on the exec stream (its execution window, as a kernel
-    trace reports it); the average is the pass's launch time. Plans that run their batch in
+    Each pass is then timed (for the roofline): `steps` whole execs run pass by pass
+    (mfft_exec_pass), the pass's launch in each between a start and a stop HIP event that
+    the runtime records around the kernel itself on the exec stream (its execution window,
+    as a kernel trace reports it); the average is the pass's launch time. (A pass launched
+    alone over and over would find the input it rereads in the last-level cache.) Plans that run their batch in
     several chunks fall back to two events around `steps` back-to-back runs of the pass
     (one launch per chunk, launch gaps included), divided by the number of launches.
     Returns (total_ms, [per-pass avg ms per launch])."""
@@ -146,17 +147,20 @@ def time_plan(plan, x, out, steps, warmup, stream):
     per_pass = []
     single = plan.info["batch_chunk"] >= plan.batch
     nchunks = -(-plan.batch // plan.info["batch_chunk"])
-    PASS_TIMING = ("kernel window: start/stop HIP events recorded around each launch (hipExtLaunchKernel)"
-                   if single else f"two HIP events around back-to-back runs of the pass ({nchunks} chunk launches "
-                                  "each), per launch")
+    PASS_TIMING = ("kernel window: start/stop HIP events recorded around the pass's launch "
+                   "(hipExtLaunchKernel) inside whole execs run pass by pass" if single else
+                   f"two HIP events around back-to-back runs of the pass ({nchunks} chunk launches each), per launch")
     pool = EventPool(2 * steps)
     for k in range(npass):
         if single:
+            # every exec runs all its passes in order (so the caches hold what they hold in
+            # an exec); only pass k's launch carries the events
             for i in range(steps):
-                rc = lib.mfft_exec_pass(plan._h, k, x.data_ptr(), out.data_ptr(), stream.cuda_stream,
-                                        pool.arr[2 * i], pool.arr[2 * i + 1])
-                if rc != 0:
-                    raise RuntimeError(lib.mfft_last_error().decode())
+                for j in range(npass):
+                    ev = (pool.arr[2 * i], pool.arr[2 * i + 1]) if j == k else (None, None)
+                    rc = lib.mfft_exec_pass(plan._h, j, x.data_ptr(), out.data_ptr(), stream.cuda_stream, *ev)
+                    if rc != 0:
+                        raise RuntimeError(lib.mfft_last_error().decode())
             torch.cuda.synchronize()
             per_pass.append(sum(pool.elapsed_ms(2 * i, 2 * i + 1) for i in range(steps)) / steps)
         else:
@@ -628,8 +632,10 @@ def conv_section(steps, warmup):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 200 execs of 65536 x 256 take ~11 ms: long enough for the clocks and caches to
+    # settle (20 read a few percent slow, each pass's launches included)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4", "c5"])
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="trace length of the serving run (c5: 30)")
     ap.add_argument("--serve-p99-ms", type=float, default=2.0, help="p99 bound of the equal-p99 comparison")
