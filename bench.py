@@ -633,8 +633,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     # 200 execs of 65536 x 256 take ~11 ms: long enough for the clocks and caches to
-    # settle (20 read a few percent slow, each pass's launches included)
-    ap.add_argument("--steps", type=int, default=200)
+    # settle (20 read a few percent slow, each pass's launches included); the 256 x 4096
+    # workload (~4 us an exec) times 1000 execs in one graph by default
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; 1000 for c1)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4", "c5"])
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="trace length of the serving run (c5: 30)")
@@ -647,6 +648,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="headline precision only")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 1000 if args.config == "c1" else 200
     global TUNE
     TUNE = not args.no_tune
     if args.gpus != 1:
