@@ -35,6 +35,24 @@ constexpr int kFastTwCols = 1024;         // the same for the strided kind (the 
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
 __host__ __device__ constexpr int tile_lds_bytes(int points) { return lds_pad(points) * 8 + 64; }
 
+// Mixed-radix digit reversal of an index whose radices are powers of two: digit i of x
+// (least significant first) is bits [sh[i], sh[i] + log2 radix) and moves to bit position
+// out[i]. n = 0 is the identity.
+struct DigitRev {
+    int n;
+    int sh[4], mask[4], out[4];
+};
+
+__host__ __device__ inline int digit_rev(int x, const DigitRev &d)
+{
+    if (d.n == 0) return x;
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < d.n) r |= ((x >> d.sh[i]) & d.mask[i]) << d.out[i];
+    return r;
+}
+
 // One workgroup transforms T lines of R points. Lines are indexed by (l0, l1, l2);
 // l0 runs fastest and a tile covers T consecutive l0. Global element (line, point)
 // lives at  l0*s0 + l1*s1 + l2*s2 + point*ps  (in complex elements) on each side.
@@ -81,6 +99,15 @@ struct TileArgs {
     const void *real_src2;     // set: the imaginary part comes from this second image instead
     int real_f64, real_b, real_c, real_h, real_w, real_pw;
     long long real_p, batch_base;
+    // fast kernels, digit-reversed input (the order of tcfftExec, gen_rev of a radix list
+    // whose first radix is 16): natural point col + (R/16) j of a line is stored at point
+    // position j + 16 * digit_rev(col, in_rev), so each lane's four first-stage operands
+    // are consecutive points. in_rev.n = 0 with rev_input = 0: natural order.
+    int rev_input;
+    DigitRev in_rev;
+    // four-step twiddle of a digit-reversed first pass: the exponent's line factor is
+    // digit_rev(line, tw_rev) (identity when tw_rev.n = 0)
+    DigitRev tw_rev;
 };
 
 // DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of

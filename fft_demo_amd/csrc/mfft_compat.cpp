@@ -3,6 +3,7 @@
 #include "../../include/mfft.h"
 #include "../../include/tcfft_half.h"
 #include "../../include/tcfft_half_2d.h"
+#include "mfft_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -21,18 +22,12 @@ int set_err(int code)
     return code;
 }
 
-// radix list of the digit-reversed order: radix-16 digits, then the 2/4/8 tail
-// (the reference's lists for 256, 512, 1024: tcfft_half.cu:1250-1270)
+// radix list of the digit-reversed order: the reference's list for every size tcfftCreate
+// knows (tcfft_half.cu:1250-1330), the planner's rule for the rest
 std::vector<int> radix_list(int n)
 {
     std::vector<int> r;
-    int e2 = 0, m = n;
-    while (m % 2 == 0) { m /= 2; ++e2; }
-    for (int i = 0; i < e2 / 4; ++i) r.push_back(16);
-    if (e2 % 4) r.push_back(1 << (e2 % 4));
-    const int odd[] = {17, 5, 3};
-    for (int p : odd)
-        while (m % p == 0) { r.push_back(p); m /= p; }
+    mfft::digit_reversal_radices(n, r);
     return r;
 }
 

@@ -59,6 +59,7 @@ struct mfft_conv2d_s {
     bool have_filters = false;
     long long bytes = 0, spec_bytes = 0;
     std::mutex mu;
+    mfft::StreamOrder order;  // streams that used H, X, Y, G
 };
 
 namespace {
@@ -84,6 +85,17 @@ bool good(int n)
     while (n % 2 == 0) n /= 2;
     return n == 1 || n == 3 || n == 5 || n == 9 || n == 15 || n == 17;
 }
+
+// One call on the operator: holds its mutex and orders its buffers (H, X, Y, G) after the
+// previous call's when that ran on another stream.
+struct OpCall {
+    mfft_conv2d op;
+    hipStream_t s;
+    std::lock_guard<std::mutex> lk;
+    hipError_t err;
+    OpCall(mfft_conv2d o, hipStream_t st) : op(o), s(st), lk(o->mu), err(o->order.acquire(st)) {}
+    ~OpCall() { (void)op->order.release(s); }
+};
 
 int plan_info_bytes(mfft_plan p)
 {
@@ -129,6 +141,7 @@ int mfft_conv2d_destroy(mfft_conv2d op)
     (void)hipFree(op->H);
     (void)hipFree(op->X);
     (void)hipFree(op->Y);
+    op->order.destroy();
     delete op;
     return MFFT_OK;
 }
@@ -215,8 +228,9 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
 int mfft_conv2d_set_filters(mfft_conv2d op, const void *filters, int real_fmt, void *stream)
 {
     if (!op || !filters) return cfail(MFFT_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> lk(op->mu);
     hipStream_t s = (hipStream_t)stream;
+    OpCall call(op, s);
+    if (call.err != hipSuccess) return cfail(MFFT_ERR_HIP, "stream order: %s", hipGetErrorString(call.err));
     const mfft_conv2d_desc &d = op->d;
     if (mfft::conv_pack_filters(filters, real_fmt == MFFT_REAL_F64, op->H, d.filters * d.channels, d.fh, d.fw, op->Ph,
                                 op->Pw, d.mode == MFFT_CONV_VALID_CORR, s) != hipSuccess)
@@ -232,8 +246,9 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
     if (!op || !in || !out) return cfail(MFFT_ERR_INVALID, "null argument");
     if (!op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
     if (fsel >= op->d.filters) return cfail(MFFT_ERR_INVALID, "filter %d of %d", fsel, op->d.filters);
-    std::lock_guard<std::mutex> lk(op->mu);
     hipStream_t s = (hipStream_t)stream;
+    OpCall call(op, s);
+    if (call.err != hipSuccess) return cfail(MFFT_ERR_HIP, "stream order: %s", hipGetErrorString(call.err));
     const mfft_conv2d_desc &d = op->d;
     int rc;
     if (mfft::plan_first_pass_fusable(op->fwd_img)) {
@@ -295,8 +310,9 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
     if (d.channels != 1) return cfail(MFFT_ERR_UNSUPPORTED, "forward_two needs a single-channel operator");
     if (filter0 < 0 || filter1 < 0 || filter0 >= d.filters || filter1 >= d.filters)
         return cfail(MFFT_ERR_INVALID, "filter index");
-    std::lock_guard<std::mutex> lk(op->mu);
     hipStream_t s = (hipStream_t)stream;
+    OpCall call(op, s);
+    if (call.err != hipSuccess) return cfail(MFFT_ERR_HIP, "stream order: %s", hipGetErrorString(call.err));
     const long long P = (long long)op->Ph * op->Pw;
     int rc;
     if (mfft::plan_first_pass_fusable(op->fwd_img)) {
@@ -368,10 +384,11 @@ int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int 
     if (op->d.mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_UNSUPPORTED, "backward of the 'valid' correlation mode only");
     if (op->spec_fmt != MFFT_FMT_C32) return cfail(MFFT_ERR_UNSUPPORTED, "backward needs fp32 spectra (spec_fp16 = 0)");
     if (grad_in && !op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
-    std::lock_guard<std::mutex> lk(op->mu);
+    hipStream_t s = (hipStream_t)stream;
+    OpCall call(op, s);
+    if (call.err != hipSuccess) return cfail(MFFT_ERR_HIP, "stream order: %s", hipGetErrorString(call.err));
     int rc = backward_setup(op);
     if (rc) return rc;
-    hipStream_t s = (hipStream_t)stream;
     const mfft_conv2d_desc &d = op->d;
     const int f64 = real_fmt == MFFT_REAL_F64;
     const long long P = (long long)op->Ph * op->Pw;

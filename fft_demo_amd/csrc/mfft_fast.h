@@ -29,6 +29,10 @@ enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2 };
 constexpr int kColsMaxR = 1024;
 constexpr int cols_lines(int R) { return R <= 256 ? 32 : 8192 / R; }
 
+// Precisions and input formats whose fast kernels also read digit-reversed lines
+// (TileArgs::rev_input): the matrix-core paths with fp16 or fp32 input.
+constexpr bool fast_rev_ok(int prec, int in_fmt) { return prec != PREC_FP32 && in_fmt != FMT_CBF16; }
+
 typedef void (*fast_fn)(TileArgs);
 fast_fn find_fast_kernel(int prec, int R, int kind, int in_fmt, int out_fmt);
 int fast_tile_lines(int R, int kind);

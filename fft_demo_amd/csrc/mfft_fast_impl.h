@@ -264,13 +264,70 @@ struct Prefetch {
     raw w[N];
 
     // FULL: every tile of the launch has all its lines (the line count is a multiple of
-    // the tile's), so the loads need no per-lane masks.
-    template <bool FULL>
+    // the tile's), so the loads need no per-lane masks. REV: the lines are stored in
+    // digit-reversed order (TileArgs::rev_input): the lane's four operands of a group,
+    // natural points col + R16 (4g + i), sit at the consecutive positions
+    // 16 digit_rev(col) + 4g + i.
+    template <bool FULL, bool REV = false>
     __device__ __forceinline__ void load(const TileArgs &a, const Origin &o)
     {
         constexpr int MAP = KIND == KIND_COLS ? MAP_LINES : MAP_POINTS;
         constexpr int R16 = G::R16;
         const raw *src = reinterpret_cast<const raw *>(a.in);
+        if constexpr (REV) {
+            static_assert(PREC != PREC_FP32, "digit-reversed loads are for the matrix-core kernels");
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+            for (int u = 0; u < G::MAXP; ++u) {
+                const int P = wave + u * G::NW;
+                int t, col;
+                slot_tc<R, G::T, MAP>(P * 16 + c, t, col);
+                const bool ok = (G::PAIRS % G::NW == 0 || P < G::PAIRS) && (FULL || t < o.nv);
+                const long long pos = 16LL * digit_rev(col, a.in_rev) + 4 * g;
+                if constexpr (KIND == KIND_COLS) {
+                    // strided lines: each point a row of its own, consecutive lines across lanes
+                    const raw *base = src + o.in_base + (long long)t * a.in_s0 + pos * a.in_ps;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        if constexpr (INF == FMT_C32)
+                            w[4 * u + i] = ok ? base[i * a.in_ps] : make_float2(0.f, 0.f);
+                        else
+                            w[4 * u + i] = ok ? base[i * a.in_ps] : 0u;
+                    }
+                } else {
+                    // contiguous lines (in_ps = 1): the four points are one 16-byte (C16) or
+                    // two 16-byte (C32) loads when the buffer is 16-byte aligned (every line
+                    // and position offset is a multiple of four points)
+                    const raw *base = src + o.in_base + (long long)t * a.in_s0 + pos;
+                    if constexpr (INF == FMT_C32) {
+                        typedef float f4v __attribute__((ext_vector_type(4)));
+                        if (a.in_vec) {
+                            const f4v z = {0.f, 0.f, 0.f, 0.f};
+                            const f4v v0 = ok ? reinterpret_cast<const f4v *>(base)[0] : z;
+                            const f4v v1 = ok ? reinterpret_cast<const f4v *>(base)[1] : z;
+                            w[4 * u + 0] = make_float2(v0.x, v0.y);
+                            w[4 * u + 1] = make_float2(v0.z, v0.w);
+                            w[4 * u + 2] = make_float2(v1.x, v1.y);
+                            w[4 * u + 3] = make_float2(v1.z, v1.w);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) w[4 * u + i] = ok ? base[i] : make_float2(0.f, 0.f);
+                        }
+                    } else {
+                        if (a.in_vec) {
+                            const u4 v = ok ? *reinterpret_cast<const u4 *>(base) : u4{0u, 0u, 0u, 0u};
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) w[4 * u + i] = v[i];
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) w[4 * u + i] = ok ? base[i] : 0u;
+                        }
+                    }
+                }
+            }
+            return;
+        }
         if constexpr (PREC == PREC_FP32) {
             int t, col;
             slot_tc<R, G::T, MAP>(threadIdx.x, t, col);
@@ -353,6 +410,16 @@ __device__ __forceinline__ float2 tw1(const TileArgs &a, const float2 *twl, unsi
     return cmulf(make_float2(th[ih], th[ih + 1]), make_float2(tl[il], tl[il + 1]));
 }
 
+// Line factor of the four-step twiddle exponent: the line index along the twiddle level,
+// digit-reversed (TileArgs::tw_rev) in the first pass of a digit-reversed plan.
+template <bool REV>
+__device__ __forceinline__ unsigned tw_line(const TileArgs &a, const Origin &o, int t)
+{
+    const int line = a.tw_level ? o.l1 : o.l0b + t;
+    if constexpr (REV) return (unsigned)digit_rev(line, a.tw_rev);
+    return (unsigned)line;
+}
+
 // ------------------------------------------------------------------ radix-16 stage (MFMA)
 // FIRST: operands are the prefetched raw inputs; LAST: results go to global memory,
 // multiplied by the four-step twiddles when TW (a.tw_N != 0), else by a.last_mul.
@@ -361,7 +428,8 @@ __device__ __forceinline__ float2 tw1(const TileArgs &a, const float2 *twl, unsi
 // from global memory (a.tw_lo_s, a.tw_hi). The two sources are separate code paths so
 // each keeps its address space (LDS reads wait on the LDS counter only, never on the
 // next tile's prefetch loads).
-template <int PREC, int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF, bool TW = false>
+template <int PREC, int R, int KIND, int MAP, bool FIRST, bool LAST, int INF, int OUTF, bool TW = false,
+          bool REV = false>
 __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Origin &o, int s, const Ops<PREC> &op,
                                        const Prefetch<PREC, R, KIND, INF> &pf, const float2 *stw,
                                        const float2 *twl)
@@ -408,7 +476,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
     if constexpr (LAST && TW && SAME_T) {
         int t, col;
         slot_tc<R, T, MAP>(wave * 16 + c, t, col);
-        const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+        const unsigned mult = tw_line<REV>(a, o, t);
         tw_step = tw1<G::TWC, true>(a, twl, mult * (unsigned)R16);
         const float inv = 1.0f / a.last_mul;  // the table's stage scale, a power of two
         tw_step.x *= inv;
@@ -436,7 +504,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
                 raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
                 float2 w[4];
                 if constexpr (TW) {
-                    const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+                    const unsigned mult = tw_line<REV>(a, o, t);
                     float2 st = tw_step;
                     if constexpr (!SAME_T) {
                         st = tw1<G::TWC, true>(a, twl, mult * (unsigned)R16);
@@ -642,7 +710,7 @@ __device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, fl
 // unscaled tables: this stage is VALU, no x16 operand factor), the tail's stage scale and
 // the output conversion. Lanes walk the points of a line (ROWS) or the lines (the
 // transposed and strided kinds), whichever is contiguous in the output.
-template <int R, int KIND, int RADIX, int NW, int OUTF, bool TW>
+template <int R, int KIND, int RADIX, int NW, int OUTF, bool TW, bool REV = false>
 __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, const Origin &o, const float2 *twl)
 {
     typedef Geo<R, KIND, NW> G;
@@ -665,7 +733,7 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
         if (t >= o.nv) continue;
         dft_tail<RADIX>(v, a.sign);
         raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
-        const unsigned mult = (unsigned)(a.tw_level ? o.l1 : o.l0b + t);
+        const unsigned mult = tw_line<REV>(a, o, t);
 #pragma unroll
         for (int q4 = 0; q4 < RADIX; q4 += 4) {
             float2 w[4];
@@ -713,7 +781,7 @@ __device__ __forceinline__ void stage_tables(const TileArgs &a, float2 *s_stw, f
     if constexpr (PREC != PREC_FP32) op.load(reinterpret_cast<const DftOps *>(a.dft_ops), threadIdx.x & 63);
 }
 
-template <int PREC, int R, int KIND, int INF, int OUTF, bool TW, bool FULL>
+template <int PREC, int R, int KIND, int INF, int OUTF, bool TW, bool FULL, bool REV = false>
 __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float2 *stw, float2 *s_tw)
 {
     typedef Geo<R, KIND, fast_waves<PREC, KIND, R>()> G;
@@ -741,7 +809,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float
     TileWalk walk;
     walk.init(a, G::T, tile < end ? tile : 0, step);
     Origin o = walk.origin(a, G::T);
-    if (tile < end) cur.template load<FULL>(a, o);
+    if (tile < end) cur.template load<FULL, REV>(a, o);
     // the tables are loaded after the first tile's operands, so the two global-memory
     // round trips overlap (a workgroup with one tile waits for one, not two)
     Ops<PREC> op;
@@ -757,7 +825,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float
         if (next < end) {
             walk.advance(a);
             on = walk.origin(a, G::T);
-            nxt.template load<FULL>(a, on);
+            nxt.template load<FULL, REV>(a, on);
         }
         int s = 1;
         if constexpr (PREC == PREC_FP32)
@@ -767,7 +835,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float
         s = 16;
         if constexpr (NS == 1) {
             // R = 16 * TAIL: the tail is the last stage and stores to global memory
-            tstage_last<R, KIND, TAIL, G::NW, OUTF, TW>(lds, a, o, s_tw);
+            tstage_last<R, KIND, TAIL, G::NW, OUTF, TW, REV>(lds, a, o, s_tw);
         } else {
             if constexpr (TAIL > 1) {
                 tstage<R, KIND, TAIL, G::NW>(lds, a, s, a.tail_mul, stw);
@@ -784,7 +852,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float
             if constexpr (PREC == PREC_FP32)
                 vstage16<R, KIND, MAP_OUT, false, true, INF, OUTF, TW>(lds, a, o, s, cur, stw, s_tw);
             else
-                mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF, TW>(lds, a, o, s, op, cur, stw, s_tw);
+                mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF, TW, REV>(lds, a, o, s, op, cur, stw, s_tw);
         }
         __syncthreads();  // the next tile's first stage overwrites LDS
         tile = next;
@@ -809,6 +877,22 @@ __global__ void __launch_bounds__((fast_waves<PREC, KIND, R>() * 64)) fft_fast(T
     static_assert(NS >= 2 || (NS == 1 && TAIL > 1), "fast kernels need R >= 32");
 
     const bool full = a.c0 % G::T == 0;
+    if constexpr (fast_rev_ok(PREC, INF)) {
+        if (a.rev_input) {
+            if (a.tw_N) {
+                if (full)
+                    fast_tiles<PREC, R, KIND, INF, OUTF, true, true, true>(a, lds, s_stw, s_tw);
+                else
+                    fast_tiles<PREC, R, KIND, INF, OUTF, true, false, true>(a, lds, s_stw, s_tw);
+            } else {
+                if (full)
+                    fast_tiles<PREC, R, KIND, INF, OUTF, false, true, true>(a, lds, s_stw, s_tw);
+                else
+                    fast_tiles<PREC, R, KIND, INF, OUTF, false, false, true>(a, lds, s_stw, s_tw);
+            }
+            return;
+        }
+    }
     if (a.tw_N) {
         if (full)
             fast_tiles<PREC, R, KIND, INF, OUTF, true, true>(a, lds, s_stw, s_tw);

@@ -4,6 +4,8 @@
 #include "../../include/mfft.h"
 #include "mfft_common.h"
 
+#include <vector>
+
 namespace mfft {
 
 // Work fused into the first pass of a plan (the streaming short-DFT kernel only): an
@@ -25,5 +27,28 @@ int exec_fused(mfft_plan p, const void *in, void *out, int64_t batch, void *stre
 
 // Set the calling thread's message returned by mfft_last_error().
 void set_last_error(const char *msg);
+
+// Orders the streams that use one device buffer (a plan's workspace, a conv operator's
+// spectra). Calls are made under the owner's mutex: acquire(s) before enqueuing work on s
+// that touches the buffer, release(s) after. While every call uses one stream nothing is
+// recorded (no cost on the common path). The first time another stream shows up the
+// device is synchronised once (the previous stream may no longer exist, so no event can
+// be recorded on it after the fact); from then on every release records an event that the
+// next acquire on a different stream waits for. Work being captured into a graph is not
+// tracked: graph launches are ordered by the caller's stream.
+struct StreamOrder {
+    hipStream_t last = nullptr;
+    bool used = false;
+    hipEvent_t ev = nullptr;  // non-null once several streams have been seen
+    hipError_t acquire(hipStream_t s);
+    hipError_t release(hipStream_t s);
+    void destroy();
+};
+
+// The radix list whose mixed-radix digit reversal (gen_rev, tcfft_half.cu:1351-1374) is the
+// input order of MFFT_ORDER_DIGIT_REVERSED plans of length n: the reference's lists for the
+// sizes tcfftCreate handles (tcfft_half.cu:1250-1330), else radix-16 digits, the 2/4/8 tail,
+// then the odd factors.
+void digit_reversal_radices(int n, std::vector<int> &out);
 
 }  // namespace mfft

@@ -85,11 +85,23 @@ static bool line_radices(int R, std::vector<int> &out)
     return true;
 }
 
-// The reference-style radix list used for the digit-reversed input order:
-// radix-16 digits first, then the 2/4/8 tail (tcfft_half.cu:1250-1330 for 512, 1024).
+// The radix list that defines the digit-reversed input order of a length-n axis.
+// The sizes tcfftCreate knows get the reference's own lists (tcfft_half.cu:1250-1330:
+// the tail after every second radix-16 digit, where its 512/1024-point passes put it);
+// every other length: radix-16 digits first, then the 2/4/8 tail, then odd factors.
 static void rev_radices(int n, std::vector<int> &out)
 {
     out.clear();
+    switch (n) {
+    case 1 << 17: out = {16, 16, 2, 16, 16}; return;
+    case 1 << 18: out = {16, 16, 2, 16, 16, 2}; return;
+    case 1 << 19: out = {16, 16, 4, 16, 16, 2}; return;
+    case 1 << 24: out = {16, 16, 16, 16, 16, 16}; return;
+    case 1 << 25: out = {16, 16, 2, 16, 16, 16, 16}; return;
+    case 1 << 26: out = {16, 16, 2, 16, 16, 2, 16, 16}; return;
+    case 1 << 27: out = {16, 16, 2, 16, 16, 2, 16, 16, 2}; return;
+    default: break;
+    }
     int m = n, e2 = 0;
     while (m % 2 == 0) { m /= 2; ++e2; }
     for (int i = 0; i < e2 / 4; ++i) out.push_back(16);
@@ -116,6 +128,51 @@ static void gen_rev(int n, const std::vector<int> &radices, std::vector<int> &re
     rev = a;
 }
 
+// DigitRev of an index whose digits, least significant first, have the radices `digits`
+// (powers of two, at most four digits): the result holds the digits in reverse order.
+static bool make_digit_rev(const std::vector<int> &digits, DigitRev &d)
+{
+    memset(&d, 0, sizeof d);
+    if (digits.size() > 4) return false;
+    int total = 0;
+    for (int r : digits) {
+        if (!is_pow2(r)) return false;
+        total += __builtin_ctz(r);
+    }
+    int sh = 0;
+    d.n = (int)digits.size();
+    for (size_t i = 0; i < digits.size(); ++i) {
+        const int w = __builtin_ctz(digits[i]);
+        d.sh[i] = sh;
+        d.mask[i] = digits[i] - 1;
+        d.out[i] = total - sh - w;
+        sh += w;
+    }
+    return true;
+}
+
+// Lines of length R stored in the digit-reversed order of `list` (position p holds natural
+// point gen_rev(list)[p]) can be read by the fast kernels' first stage when the list starts
+// with 16 and the rest reverses with a DigitRev q: natural point col + (R/16) j is then at
+// position j + 16 q(col). Fills q and checks that identity on every point.
+static bool fast_rev_spec(int R, const std::vector<int> &list, DigitRev &q)
+{
+    if (R < 64 || !is_pow2(R) || list.empty() || list[0] != 16) return false;
+    long long prod = 1;
+    for (int r : list) prod *= r;
+    if (prod != R) return false;
+    std::vector<int> col_digits(list.rbegin(), list.rend() - 1);  // r_{m-1}, ..., r_1
+    if (!make_digit_rev(col_digits, q)) return false;
+    std::vector<int> rev;
+    gen_rev(R, list, rev);
+    const int R16 = R / 16;
+    for (int pos = 0; pos < R; ++pos) {
+        const int n = rev[pos];
+        if (pos != n / R16 + 16 * digit_rev(n % R16, q)) return false;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ plan structures
 struct Level {
     long long count, in_stride, out_stride;
@@ -132,6 +189,10 @@ struct PassSpec {
     int tw_level = -1;  // index into lines
     int src = BUF_IN, dst = BUF_OUT;
     int perm_axis = -1; // axis whose digit-reversal is fused into this pass's loads
+    // the pass's input lines are stored in the digit-reversed order of this radix list
+    // (gen_rev, tcfft_half.cu:1351-1374); empty = natural order
+    std::vector<int> rev_list;
+    DigitRev rev_tw{};  // four-step twiddle line factor of a digit-reversed first pass
     long long work_stride = 0;  // workspace elements per transform when padded (0 = dense)
     // derived
     std::vector<int> radix;
@@ -148,6 +209,8 @@ struct PassSpec {
     int tw_bits = 0;
     float2 *stage_tw = nullptr;
     int *perm = nullptr;
+    bool rev_fast = false;  // the fast kernel reads the digit-reversed lines (rev_q)
+    DigitRev rev_q{};
 };
 
 struct mfft_plan_s {
@@ -165,7 +228,9 @@ struct mfft_plan_s {
     double out_scale = 1.0;
     std::vector<void *> allocs;
     std::vector<int> rev_axis[2];
+    std::vector<int> rev_list_axis[2];  // the radix lists rev_axis was generated from
     std::mutex exec_mu;       // the workspace is per plan
+    StreamOrder ws_order;     // streams that used the plan's own workspace
 };
 
 static int fmt_bytes(int fmt) { return fmt == FMT_C32 ? 8 : 4; }
@@ -212,8 +277,11 @@ static bool narrow_mid(const mfft_desc *d)
 }
 
 // narrow: the intermediates between passes are 16-bit (an extra pass costs 8 bytes a point)
+// rev_cols: a digit-reversed strided axis may still run in one pass of the strided
+// matrix-core kernel (it reads the reversed order itself)
 static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, long long ps_out,
-                     std::vector<Level> lines, int src, int dst, int perm_axis, int split_hint, bool narrow)
+                     std::vector<Level> lines, int src, int dst, int perm_axis, int split_hint, bool narrow,
+                     bool rev_cols = false)
 {
     // odd * 2^a (2176 = 17 x 128): the odd factor in a streaming pass, the power of two on
     // the matrix-core kernels, instead of one generic mixed-radix tile pass.
@@ -221,7 +289,7 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     const long long odd = odd_part(L);
     const bool odd_split = perm_axis < 0 && split_hint != 1 && odd > 1 && small_odd(odd) && L / odd >= 16 &&
                            L / odd <= kMaxTile;
-    const bool fast_cols = perm_axis < 0 && split_hint <= 1;  // no split asked for
+    const bool fast_cols = (perm_axis < 0 || rev_cols) && split_hint <= 1;  // no split asked for
     if (!odd_split && single_pass_ok((int)std::min<long long>(L, 1 << 30), ps_in, lines, fast_cols)) {
         PassSpec p;
         p.R = (int)L;
@@ -312,11 +380,77 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, a_dst, dst, -1, 1, narrow);
 }
 
+// Digit-reversed input (tcfftExec's order, tcfft_half.cu:1177-1233) of a 1D length longer
+// than one tile: N = L1 L2, split after the first m radices of the list. Position
+// p = p_lo + L1 p_hi holds natural element n2 + L2 n1 with n1 = gen_rev(list[:m])[p_lo] and
+// n2 = gen_rev(list[m:])[p_hi], so
+//   pass A: each contiguous block p_hi is a digit-reversed line of list[:m]: DFT_L1 over n1,
+//           times W_N^(n2 k1) (n2 = the digit reversal of p_hi), stored transposed at
+//           k1 L2 + p_hi;
+//   pass B: row k1 of the workspace is a digit-reversed line of list[m:] (position p_hi holds
+//           n2): DFT_L2, stored transposed at k1 + L1 k2, natural order.
+// Both passes read contiguous lines. The halves must start with a radix-16 digit and run
+// on the fast kernels, and L1, L2 are kept to 256..1024 points (longer lines hold too few
+// per tile for the transposed stores).
+static int plan_reversed_1d(mfft_plan_s *p)
+{
+    const std::vector<int> &list = p->rev_list_axis[0];
+    const long long N = p->n;
+    int best = -1;
+    double bestd = 1e9;
+    long long L1 = 1;
+    for (size_t m = 1; m < list.size(); ++m) {
+        L1 *= list[m - 1];
+        const long long L2 = N / L1;
+        if (list[0] != 16 || list[m] != 16 || L1 < 256 || L1 > 1024 || L2 < 256 || L2 > 1024) continue;
+        const double dd = std::fabs(std::log2((double)L1) - std::log2((double)L2));
+        if (dd < bestd) {
+            bestd = dd;
+            best = (int)m;
+        }
+    }
+    if (best < 0) return fail(MFFT_ERR_UNSUPPORTED, "digit-reversed input of length %lld in two passes", N);
+    std::vector<int> lo(list.begin(), list.begin() + best), hi(list.begin() + best, list.end());
+    L1 = 1;
+    for (int r : lo) L1 *= r;
+    const long long L2 = N / L1;
+    PassSpec a;
+    a.R = (int)L1;
+    a.ps_in = 1;
+    a.ps_out = L2;
+    a.lines = {Level{L2, L1, 1, false}, Level{p->batch, N, N, true}};
+    a.tw_N = (int)N;
+    a.tw_level = 0;
+    a.src = BUF_IN;
+    a.dst = BUF_WORK;
+    a.rev_list = lo;
+    if (!make_digit_rev(hi, a.rev_tw)) return fail(MFFT_ERR_UNSUPPORTED, "twiddle line reversal");
+    {
+        std::vector<int> rh;
+        gen_rev((int)L2, hi, rh);
+        for (int ph = 0; ph < (int)L2; ++ph)
+            if (rh[ph] != digit_rev(ph, a.rev_tw)) return fail(MFFT_ERR_UNSUPPORTED, "twiddle line reversal");
+    }
+    PassSpec b;
+    b.R = (int)L2;
+    b.ps_in = 1;
+    b.ps_out = L1;
+    b.lines = {Level{L1, L2, 1, false}, Level{p->batch, N, N, true}};
+    b.src = BUF_WORK;
+    b.dst = BUF_OUT;
+    b.rev_list = hi;
+    p->passes.push_back(a);
+    p->passes.push_back(b);
+    return MFFT_OK;
+}
+
 // Choose tile geometry and kernel for every pass.
 static int finish_passes(mfft_plan_s *p)
 {
     for (auto &ps : p->passes) {
         if (!line_radices(ps.R, ps.radix)) return fail(MFFT_ERR_UNSUPPORTED, "line length %d", ps.R);
+        if (ps.perm_axis >= 0 && ps.rev_list.empty()) ps.rev_list = p->rev_list_axis[ps.perm_axis];
+        const bool rev = !ps.rev_list.empty();
         // merge adjacent levels that are contiguous on both sides (never the batch level)
         // choose l0
         int unit_in = -1, unit_out = -1;
@@ -328,10 +462,13 @@ static int finish_passes(mfft_plan_s *p)
         bool pow2 = is_pow2(ps.R) && ps.R >= 16;
         const int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
         const int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+        // digit-reversed lines on the fast kernels: the list starts with 16 and the rest
+        // reverses digit by digit (every reference list does)
+        ps.rev_fast = rev && fast_rev_ok(p->d.precision, in_fmt) && fast_rev_spec(ps.R, ps.rev_list, ps.rev_q);
         // strided lines longer than a 16-line tile kernel holds: the strided matrix-core
         // kernel when it exists for the formats (16 or 32 lines of up to kColsMaxR points)
         const bool cols_long = strided && unit_in >= 0 && ps.R * 16 > kMaxTile && pow2 && ps.R <= kColsMaxR &&
-                               ps.perm_axis < 0 && ps.lines[unit_in].out_stride == 1 &&
+                               (!rev || ps.rev_fast) && ps.lines[unit_in].out_stride == 1 &&
                                find_fast_kernel(p->d.precision, ps.R, KIND_COLS, in_fmt, out_fmt) != nullptr;
         if (strided && unit_in >= 0 && (ps.R * 16 <= kMaxTile || cols_long)) {
             ps.l0 = unit_in;
@@ -354,7 +491,7 @@ static int finish_passes(mfft_plan_s *p)
         }
         ps.generic = !pow2;
         // short strided DFTs: streaming kernel, one thread per line
-        if (ps.perm_axis < 0 && ps.ps_in != 1 &&
+        if (!rev && ps.ps_in != 1 &&
             (ps.R == 2 || ps.R == 4 || ps.R == 8 || ps.R == 16 || small_odd(ps.R))) {
             const void *fn = find_small_kernel(ps.R, in_fmt, out_fmt);
             if (fn) {
@@ -366,8 +503,8 @@ static int finish_passes(mfft_plan_s *p)
                 ps.radix.assign(1, ps.R);
             }
         }
-        // fast kernel: power of two 64..4096, no fused permutation
-        if (!ps.small && pow2 && ps.R >= 64 && ps.R <= kMaxTile && ps.perm_axis < 0) {
+        // fast kernel: power of two 64..4096, natural order or digit-reversed lines it reads
+        if (!ps.small && pow2 && ps.R >= 64 && ps.R <= kMaxTile && (!rev || ps.rev_fast)) {
             int kind = -1, l0 = -1;
             if (ps.ps_in == 1 && ps.ps_out == 1) {
                 kind = KIND_ROWS;
@@ -406,6 +543,9 @@ static int finish_passes(mfft_plan_s *p)
                 ps.radix = r2;
             }
         }
+        if (ps.fast_kind < 0) ps.rev_fast = false;
+        if (rev && !ps.rev_fast && ps.perm_axis < 0)
+            return fail(MFFT_ERR_UNSUPPORTED, "digit-reversed %d-point lines need the matrix-core kernels", ps.R);
         if (ps.fast_kind >= 0 || ps.small) {
             // l0 chosen above
         } else if (!ps.generic) {
@@ -479,7 +619,7 @@ static int build_tables(mfft_plan_s *p)
             }
         }
         if (ps.small) ps.last_mul = p->d.scale == MFFT_SCALE_STAGE ? 1.0f / ps.R : 1.0f;
-        if (ps.perm_axis >= 0 && p->d.in_order == MFFT_ORDER_DIGIT_REVERSED) {
+        if (ps.perm_axis >= 0 && p->d.in_order == MFFT_ORDER_DIGIT_REVERSED && !ps.rev_fast) {
             const std::vector<int> &rev = p->rev_axis[ps.perm_axis];
             if ((int)rev.size() != ps.R) return fail(MFFT_ERR_INVALID, "permutation size");
             if (alloc_dev(p, (void **)&ps.perm, sizeof(int) * ps.R)) return MFFT_ERR_NOMEM;
@@ -608,6 +748,9 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         a.tw_lo_s = ps.tw_lo_s;
         a.last_mul = ps.last_mul;
         a.tail_mul = ps.tail_mul;
+        a.rev_input = ps.rev_fast ? 1 : 0;
+        a.in_rev = ps.rev_q;
+        a.tw_rev = ps.rev_tw;
         long long ntiles = ((a.c0 + a.T - 1) / a.T) * (long long)a.c1 * a.c2;
         if (ntiles <= 0) continue;
         if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
@@ -683,14 +826,17 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     p->batch = batch;
     p->d = *d;
     p->total = n;
-    if (d->in_order == MFFT_ORDER_DIGIT_REVERSED) {
-        std::vector<int> rad;
-        rev_radices(n, rad);
-        gen_rev(n, rad, p->rev_axis[0]);
+    const bool rev = d->in_order == MFFT_ORDER_DIGIT_REVERSED;
+    if (rev) {
+        rev_radices(n, p->rev_list_axis[0]);
+        gen_rev(n, p->rev_list_axis[0], p->rev_axis[0]);
     }
-    std::vector<Level> lines{Level{batch, n, n, true}};
-    rc = plan_axis(p->passes, n, 1, 1, lines, BUF_IN, BUF_OUT,
-                   d->in_order == MFFT_ORDER_DIGIT_REVERSED ? 0 : -1, d->split_hint, narrow_mid(d));
+    if (rev && n > kMaxTile) {
+        rc = plan_reversed_1d(p);
+    } else {
+        std::vector<Level> lines{Level{batch, n, n, true}};
+        rc = plan_axis(p->passes, n, 1, 1, lines, BUF_IN, BUF_OUT, rev ? 0 : -1, d->split_hint, narrow_mid(d));
+    }
     if (!rc) rc = finalize_plan(p);
     if (rc) {
         std::string msg = g_err;
@@ -721,12 +867,19 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
     p->total = (long long)nx * ny;
     bool rev = d->in_order == MFFT_ORDER_DIGIT_REVERSED;
     if (rev) {
-        std::vector<int> rad;
-        rev_radices(nx, rad);
-        gen_rev(nx, rad, p->rev_axis[0]);
-        rev_radices(ny, rad);
-        gen_rev(ny, rad, p->rev_axis[1]);
+        rev_radices(nx, p->rev_list_axis[0]);
+        gen_rev(nx, p->rev_list_axis[0], p->rev_axis[0]);
+        rev_radices(ny, p->rev_list_axis[1]);
+        gen_rev(ny, p->rev_list_axis[1], p->rev_axis[1]);
     }
+    // the column pass reads the x order itself when its kernel is the strided matrix-core
+    // one (then up to 1024 points stay one pass); its input is the rows' intermediate
+    DigitRev qx;
+    const int col_in_fmt = d->mid_fmt >= 0                          ? d->mid_fmt
+                           : d->precision == MFFT_PREC_FP16 ? MFFT_FMT_C16
+                           : d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16
+                                                            : MFFT_FMT_C32;
+    const bool rev_cols = rev && fast_rev_ok(d->precision, col_in_fmt) && fast_rev_spec(nx, p->rev_list_axis[0], qx);
     long long NN = (long long)nx * ny;
     // rows (y axis, contiguous): lines = (x, b)
     std::vector<Level> rows{Level{nx, ny, ny, false}, Level{batch, NN, NN, true}};
@@ -737,7 +890,8 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
         // split_hint > 1: the first-pass length of the column axis when it is split
         const int col_hint = d->split_hint > 1 ? d->split_hint : odd_hint;
         std::vector<Level> cols{Level{ny, 1, 1, false}, Level{batch, NN, NN, true}};
-        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, col_hint, narrow_mid(d));
+        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, col_hint, narrow_mid(d),
+                       rev_cols);
     }
     if (!rc) rc = finalize_plan(p);
     if (rc) {
@@ -755,6 +909,7 @@ int mfft_plan_destroy(mfft_plan plan)
     if (!plan) return MFFT_OK;
     (void)hipDeviceSynchronize();
     for (void *ptr : plan->allocs) (void)hipFree(ptr);
+    plan->ws_order.destroy();
     delete plan;
     return MFFT_OK;
 }
@@ -812,13 +967,24 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
     long long ib = p->total * fmt_bytes(p->d.in_fmt), ob = p->total * fmt_bytes(p->d.out_fmt);
     if (batch < 0) batch = p->batch;
     if (!work && !p->work && p->work_bytes > 0) return fail(MFFT_ERR_INVALID, "plan needs a workspace (mfft_exec_ex)");
-    int evi = 0;
-    for (long long b0 = 0; b0 < batch; b0 += p->chunk) {
+    // the plan's own workspace: an exec on another stream than the previous one runs
+    // after it (two streams sharing p->work would otherwise overwrite each other's
+    // intermediates between the passes)
+    const bool own_ws = !work && p->work;
+    if (own_ws) HIP_TRY(p->ws_order.acquire(s));
+    int evi = 0, rc = MFFT_OK;
+    for (long long b0 = 0; b0 < batch && !rc; b0 += p->chunk) {
         long long nb = std::min(p->chunk, batch - b0);
-        int rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work,
-                            fz, b0, only, kstart, kstop);
-        if (rc) return rc;
+        rc = exec_chunk(p, (const char *)in + b0 * ib, (char *)out + b0 * ob, nb, s, ev, evi, work ? work : p->work, fz,
+                        b0, only, kstart, kstop);
     }
+    if (own_ws) {
+        std::string keep = g_err;
+        hipError_t e = p->ws_order.release(s);  // also after a partial enqueue
+        if (!rc && e != hipSuccess) return fail(MFFT_ERR_HIP, "workspace order: %s", hipGetErrorString(e));
+        if (rc) g_err = keep;
+    }
+    if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
     return MFFT_OK;
 }
@@ -1072,6 +1238,37 @@ const char *mfft_version(void) { return "mfft 0.1 (gfx950 matrix-core FFT)"; }
 namespace mfft {
 
 void set_last_error(const char *msg) { g_err = msg ? msg : ""; }
+
+void digit_reversal_radices(int n, std::vector<int> &out) { rev_radices(n, out); }
+
+static bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+hipError_t StreamOrder::acquire(hipStream_t s)
+{
+    if (!used || s == last || capturing(s)) return hipSuccess;
+    if (ev) return hipStreamWaitEvent(s, ev, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    return e;
+}
+
+hipError_t StreamOrder::release(hipStream_t s)
+{
+    if (capturing(s)) return hipSuccess;
+    last = s;
+    used = true;
+    return ev ? hipEventRecord(ev, s) : hipSuccess;
+}
+
+void StreamOrder::destroy()
+{
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
+}
 
 bool plan_first_pass_fusable(mfft_plan p)
 {

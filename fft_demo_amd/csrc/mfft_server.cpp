@@ -88,6 +88,7 @@ struct mfft_server_s {
     mfft_batcher batcher[2] = {nullptr, nullptr};  // forward, inverse
     std::map<std::tuple<int, int, int, int>, mfft_plan> plans;  // (kind, nx, ny, dir)
     std::vector<FilterHost> filters;
+    std::vector<std::pair<int, int>> stale_classes;  // (fh & 1, fw & 1) changed since their operators were built; under mu
     struct mfft_server_convs *conv = nullptr;
     std::mutex mu;
     std::condition_variable cv;
@@ -107,6 +108,15 @@ struct mfft_server_s {
 };
 
 namespace {
+
+// Transform lengths the library plans: n >= 2 with prime factors in {2, 3, 5, 17}.
+bool smooth_length(int n)
+{
+    if (n < 2) return false;
+    for (int p : {2, 3, 5, 17})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
 
 long long io_bytes(const mfft_server_s *S, const mfft_request &r)
 {
@@ -394,6 +404,7 @@ void scheduler(mfft_server_s *S)
     std::vector<Req *> small;
     std::vector<mfft_batch_item> items;
     std::map<ConvSlot *, std::vector<std::pair<Req *, int>>> convq;  // admitted conv requests per operator
+    std::vector<std::pair<int, int>> stale;
     while (true) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
@@ -405,7 +416,22 @@ void scheduler(mfft_server_s *S)
                 S->pending.push_back(S->incoming.front());
                 S->incoming.pop_front();
             }
+            stale.swap(S->stale_classes);
         }
+        // filters added since: drop the operators of their classes (nothing runs on them --
+        // filters are only added while no request is outstanding) before anything is admitted
+        for (auto &cls : stale)
+            for (auto it = S->conv->slots.begin(); it != S->conv->slots.end();) {
+                if (std::get<2>(it->first) == cls.first && std::get<3>(it->first) == cls.second) {
+                    mfft_conv2d_info inf;
+                    if (mfft_conv2d_info_get(it->second.op, &inf) == MFFT_OK) S->static_bytes -= inf.device_bytes;
+                    mfft_conv2d_destroy(it->second.op);
+                    it = S->conv->slots.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        stale.clear();
         const auto t_busy = std::chrono::steady_clock::now();
         // retire: query each event once; requests of one coalesced or paired launch share
         // their event and retire together (its owner returns it to the pool)
@@ -602,9 +628,19 @@ int mfft_server_add_filter(mfft_server S, int filter_id, const float *weights, i
     if (!S || !weights || fh < 1 || fw < 1) return sfail(MFFT_ERR_INVALID, "bad argument");
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->incoming.empty() || !S->pending.empty() || S->submitted != S->completed)
-        return sfail(MFFT_ERR_INVALID, "filters are registered before requests are submitted");
+        return sfail(MFFT_ERR_INVALID, "filters are registered while no request is outstanding");
     FilterHost f{filter_id, fh, fw, std::vector<float>(weights, weights + (size_t)fh * fw)};
-    S->filters.push_back(f);
+    // A conv operator already built for this filter's size-parity class holds the spectra
+    // of the class as it was: the scheduler drops those operators before its next
+    // admission, and the next request of the class rebuilds them with every filter.
+    S->stale_classes.push_back({fh & 1, fw & 1});
+    auto old = std::find_if(S->filters.begin(), S->filters.end(), [&](const FilterHost &g) { return g.id == filter_id; });
+    if (old != S->filters.end()) {
+        S->stale_classes.push_back({old->fh & 1, old->fw & 1});  // a re-registered id leaves its old class
+        *old = f;
+    } else {
+        S->filters.push_back(f);
+    }
     return MFFT_OK;
 }
 
@@ -615,6 +651,20 @@ int mfft_server_submit(mfft_server S, const mfft_request *r)
     if ((r->kind == MFFT_REQ_FFT1D && (r->batch < 1 || r->batch > S->d.max_batch_1d)) ||
         (r->kind == MFFT_REQ_FFT2D && (r->batch < 1 || r->batch > S->d.max_batch_2d)))
         return sfail(MFFT_ERR_INVALID, "batch outside the server's plans");
+    if (r->direction != 0 && r->direction != MFFT_FORWARD && r->direction != MFFT_INVERSE)
+        return sfail(MFFT_ERR_INVALID, "direction %d", r->direction);
+    // Shapes are checked here, before the request can join a coalesced launch: the batcher
+    // rejects a whole launch at its first unsupported length, which would fail every small
+    // request admitted with it.
+    if (r->kind == MFFT_REQ_FFT1D && !smooth_length(r->n))
+        return sfail(MFFT_ERR_INVALID, "1D length %d: not 2^a 3^b 5^c 17^d >= 2", r->n);
+    if (r->kind == MFFT_REQ_FFT2D && (!smooth_length(r->nx) || !smooth_length(r->ny)))
+        return sfail(MFFT_ERR_INVALID, "2D shape %d x %d: sides must be 2^a 3^b 5^c 17^d >= 2", r->nx, r->ny);
+    if (r->kind == MFFT_REQ_CONV2D) {
+        if (r->nx < 1 || r->ny < 1) return sfail(MFFT_ERR_INVALID, "conv image %d x %d", r->nx, r->ny);
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!find_filter(S, r->filter_id)) return sfail(MFFT_ERR_INVALID, "filter %d is not registered", r->filter_id);
+    }
     Req *q = new Req;
     q->r = *r;
     if (q->r.direction == 0) q->r.direction = MFFT_FORWARD;

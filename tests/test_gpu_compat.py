@@ -26,17 +26,30 @@ def _lib():
     return L, L.load()
 
 
+# The reference's radix lists (tcfftCreate, tcfft_half.cu:1250-1330; radices default to 16,
+# tcfft_half.h:10), which fix the digit-reversed input order its callers produce; sizes the
+# reference has no case for take radix-16 digits then the tail (4096, 65536).
+REF_RADICES = {
+    256: [16, 16], 512: [16, 16, 2], 1024: [16, 16, 4], 4096: [16, 16, 16], 65536: [16, 16, 16, 16],
+    131072: [16, 16, 2, 16, 16], 262144: [16, 16, 2, 16, 16, 2], 524288: [16, 16, 4, 16, 16, 2],
+}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,batch", [(256, 4096), (512, 64), (1024, 32), (4096, 8), (65536, 4), (131072, 2)])
+@pytest.mark.parametrize("n,batch", [(256, 4096), (512, 64), (1024, 32), (4096, 8), (65536, 4), (131072, 2),
+                                     (262144, 2), (524288, 1)])
 def test_tcfft_exec_digit_reversed_in_place(dev, n, batch):
+    """Input permuted with gen_rev on the reference's own radix list for the size (not the
+    handle's), as the reference's setup() does (tcfft_half.cu:1385-1391)."""
     import torch
 
     L, lib = _lib()
     h = L.TcfftHandle()
     lib.tcfftCreate(ctypes.byref(h), n, batch)
     assert lib.tcfftGetLastError() == 0, lib.mfft_last_error()
-    radices = list(h.radices[: h.n_radices])
-    assert np.prod(radices) == n
+    radices = REF_RADICES[n]
+    assert list(h.radices[: h.n_radices]) == radices
+    assert list(h.radices[h.n_radices:]) == [16] * (9 - h.n_radices)
     x = complex_input((batch, n), 100 + n % 97)
     rev = oracle.gen_rev(n, radices)
     buf = to_device(x[:, rev], torch.float16, dev)  # setup()'s host permutation
@@ -148,3 +161,52 @@ def test_conv_cpp_program_reference_interface():
     r = subprocess.run([exe, "256", "13", "16"], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp16", "split_fp16"])
+@pytest.mark.parametrize("n,batch", [(256, 64), (512, 32), (1024, 16), (4096, 8), (65536, 4), (131072, 2),
+                                     (262144, 2), (524288, 1)])
+def test_digit_reversed_plans_on_fast_kernels(dev, prec, n, batch):
+    """MFFT_ORDER_DIGIT_REVERSED plans (tcfftExec's input order, gen_rev of the reference's
+    radix list) run on the matrix-core kernels, which read the reversed order in their
+    first-stage loads: one pass up to 4096 points, two contiguous-line passes above (the
+    first with its four-step twiddle on the digit-reversed line index). Unnormalised, as
+    tcfftExec; tolerance fp16 rel-L2 4e-3 / max 2e-2, split fp16 1e-3 / 4e-3."""
+    import fft_demo_amd as F
+
+    tol = {"fp16": (4e-3, 2e-2), "split_fp16": (1e-3, 4e-3)}[prec]
+    plan = F.Plan((n,), batch, precision=prec, in_order=1, scale=0)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    print(f"n={n} {prec}: passes {kinds}")
+    assert all(k in ("fast_rows", "fast_rows_tr") for k, _ in kinds), kinds
+    assert len(kinds) == (1 if n <= 4096 else 2)
+    rev = oracle.gen_rev(n, REF_RADICES[n])
+    assert np.array_equal(plan.input_permutation(), rev)
+    x = complex_input((batch, n), 200 + n % 89)
+    y = from_device(plan.exec(to_device(x[:, rev], plan.in_dtype(), dev))) / plan.out_scale
+    l2, mx = rel_errors(y, oracle.fft(x))
+    print(f"  rel_l2={l2:.2e} rel_max={mx:.2e}")
+    assert l2 <= tol[0] and mx <= tol[1]
+    plan.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(256, 256), (512, 256), (256, 1024), (1024, 1024), (64, 128)])
+def test_digit_reversed_2d_plans_on_fast_kernels(dev, nx, ny):
+    """2D digit-reversed plans (rows by rev_y, columns by rev_x, tcfft_half_2d.cu:1264-1271):
+    the row pass and the strided column pass read the reversed orders themselves."""
+    import fft_demo_amd as F
+
+    b = 2
+    plan = F.Plan((nx, ny), b, precision="fp16", in_order=1, scale=0)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    print(f"{nx}x{ny}: passes {kinds}")
+    assert kinds == [("fast_rows", ny), ("fast_cols", nx)], kinds
+    rx, ry = plan.input_permutation(0), plan.input_permutation(1)
+    x = complex_input((b, nx, ny), 31)
+    y = from_device(plan.exec(to_device(x[:, rx][:, :, ry], plan.in_dtype(), dev))) / plan.out_scale
+    l2, mx = rel_errors(y, oracle.fft2d(x))
+    print(f"  rel_l2={l2:.2e} rel_max={mx:.2e}")
+    assert l2 <= 4e-3 and mx <= 2e-2
+    plan.close()

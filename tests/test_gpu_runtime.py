@@ -94,3 +94,56 @@ def test_exec_ex_partial_batch_own_workspace(dev):
     l2, _ = rel_errors(y[:5], oracle.fft(x[:5]))
     assert l2 <= 1e-3
     assert float(out[5:].abs().max()) == 0.0  # transforms past the requested batch untouched
+
+
+@pytest.mark.gpu
+def test_two_streams_share_plan_workspace(dev):
+    """One plan (two passes through its own workspace) called from two streams with no
+    synchronisation in between, alternating several times: each exec is ordered after the
+    previous one's use of the workspace, so every output equals the single-stream result
+    bit for bit."""
+    import torch
+
+    import fft_demo_amd as F
+
+    n, b = 65536, 8
+    plan = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16")
+    assert plan.info["workspace_bytes"] > 0
+    xs = [to_device(complex_input((b, n), 60 + k), torch.float16, dev) for k in range(4)]
+    want = [plan.exec(x).clone() for x in xs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)]
+    outs = [torch.zeros_like(w) for w in want]
+    for rep in range(3):
+        for k, x in enumerate(xs):
+            plan.exec(x, outs[k], stream=streams[k % 2])
+        torch.cuda.synchronize()
+        for k in range(4):
+            assert torch.equal(outs[k], want[k]), (rep, k)
+            outs[k].zero_()
+    plan.close()
+
+
+@pytest.mark.gpu
+def test_two_streams_share_conv_operator(dev):
+    """One conv operator (its spectra buffers) called from two streams back to back: results
+    equal the single-stream ones bit for bit."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    h, w, k = 120, 96, 7
+    op = Conv2d(2, 1, 1, h, w, k, k, mode="same")
+    op.set_filters(torch.from_numpy(oracle.uniform(k * k, 8, 0.0, 1.0).reshape(1, 1, k, k)).to(dev, torch.float32))
+    xs = [torch.from_numpy(oracle.uniform(2 * h * w, 90 + i, 0.0, 1.0).reshape(2, 1, h, w)).to(dev, torch.float32)
+          for i in range(4)]
+    want = [op.forward(x).clone() for x in xs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)]
+    outs = [torch.zeros_like(t) for t in want]
+    for i, x in enumerate(xs):
+        op.forward(x, outs[i], stream=streams[i % 2])
+    torch.cuda.synchronize()
+    for i in range(4):
+        assert torch.equal(outs[i], want[i]), i
+    op.close()

@@ -181,3 +181,75 @@ def test_server_plain_mode_fp32(dev):
                 l2, _ = rel_errors(y, ref)
             print(f"plain kind {kind} n={n} {nx}x{ny} b={b} rel_l2={l2:.2e}")
             assert l2 <= 1e-5
+
+
+@pytest.mark.gpu
+def test_server_rejects_unsupported_lengths_at_submit(dev):
+    """A 1D request of length 7 (a prime factor outside {2, 3, 5, 17}) or 1 is refused at
+    submission, so it can never join a coalesced work-queue launch; the valid small
+    requests submitted around it all complete against the oracle."""
+    import torch
+
+    from fft_demo_amd import serve
+
+    with serve.Server(precision="split", max_batch_1d=64) as S:
+        xs, outs = [], []
+        bad = torch.zeros(2 * 7 * 4, dtype=torch.float16, device=dev)
+        for i, n in enumerate((256, 1024, 48, 4096)):
+            x = complex_input((3, n), 700 + i)
+            xs.append(x)
+            xd = to_device(x, torch.float16, dev)
+            outs.append((xd, torch.empty_like(xd)))
+            S.submit(serve.KIND_1D, xd.data_ptr(), outs[-1][1].data_ptr(), tag=i, n=n, batch=3)
+            for badn in (7, 1):
+                with pytest.raises(serve.ServerError, match="length"):
+                    S.submit(serve.KIND_1D, bad.data_ptr(), bad.data_ptr(), tag=100 + i, n=badn, batch=4)
+        with pytest.raises(serve.ServerError, match="shape"):
+            S.submit(serve.KIND_2D, bad.data_ptr(), bad.data_ptr(), tag=200, nx=7, ny=4, batch=1)
+        with pytest.raises(serve.ServerError, match="filter"):
+            S.submit(serve.KIND_CONV, bad.data_ptr(), bad.data_ptr(), tag=201, nx=8, ny=8, filter_id=5)
+        S.wait_idle(60.0)
+        done = {c.tag: c for c in S.poll()}
+        assert sorted(done) == [0, 1, 2, 3] and all(c.status == 0 for c in done.values())
+        torch.cuda.synchronize()
+        for i, x in enumerate(xs):
+            l2, mx = rel_errors(from_device(outs[i][1]) / done[i].out_scale, oracle.fft(x))
+            assert l2 <= 1e-3 and mx <= 4e-3, (i, l2, mx)
+
+
+@pytest.mark.gpu
+def test_server_filter_added_after_conv_requests(dev):
+    """A filter registered after conv requests have built the operator of its size-parity
+    class (and a filter id registered again with new weights) is used by the next requests:
+    the operator is rebuilt with the class's current filters. Results against the oracle."""
+    import torch
+
+    from fft_demo_amd import serve
+
+    h, w = 96, 128
+    img = oracle.uniform(h * w, 810, 0.0, 1.0).reshape(h, w).astype(np.float32)
+    f3 = oracle.uniform(9, 811, 0.0, 1.0).reshape(3, 3).astype(np.float32)
+    f5 = oracle.uniform(25, 812, 0.0, 1.0).reshape(5, 5).astype(np.float32)
+    f3b = oracle.uniform(9, 813, -1.0, 1.0).reshape(3, 3).astype(np.float32)
+    with serve.Server(precision="split") as S:
+        S.add_filter(0, f3)
+        xd = torch.from_numpy(img).to(dev)
+        out = torch.empty_like(xd)
+
+        def run(fid, filt, tag):
+            S.submit(serve.KIND_CONV, xd.data_ptr(), out.data_ptr(), tag=tag, nx=h, ny=w, filter_id=fid)
+            S.wait_idle(60.0)
+            (c,) = S.poll()
+            assert c.status == 0, c
+            torch.cuda.synchronize()
+            l2, mx = oracle.err_l2max(out.double().cpu().numpy(), oracle.conv2d_same(img, filt))
+            print(f"filter {fid} {filt.shape}: rel_l2={l2:.2e} rel_max={mx:.2e}")
+            assert l2 <= 1e-4 and mx <= 1e-3
+
+        run(0, f3, 1)
+        S.add_filter(1, f5)  # same (odd, odd) class, larger frame
+        run(1, f5, 2)
+        run(0, f3, 3)
+        S.add_filter(0, f3b)  # new weights for an existing id
+        run(0, f3b, 4)
+        run(1, f5, 5)
