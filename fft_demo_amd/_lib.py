@@ -175,6 +175,7 @@ SIGNATURES.update({
     "mfft_batcher_exec": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, _vp]),
     "mfft_batcher_destroy": (ctypes.c_int, [_vp]),
     "mfft_exec_ex": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "mfft_exec_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp]),
 })
 
 

@@ -28,6 +28,10 @@ int exec_fused(mfft_plan p, const void *in, void *out, int64_t batch, void *stre
 // Set the calling thread's message returned by mfft_last_error().
 void set_last_error(const char *msg);
 
+// The device address, on the current device, of the array mfft_exec_noalloc uses as its
+// workspace (resolved with hipGetSymbolAddress on first use per device, then cached).
+int noalloc_scratch(void **out);
+
 // Orders the streams that use one device buffer (a plan's workspace, a conv operator's
 // spectra). Calls are made under the owner's mutex: acquire(s) before enqueuing work on s
 // that touches the buffer, release(s) after. While every call uses one stream nothing is

@@ -795,6 +795,17 @@ int mfft_desc_default(mfft_desc *d)
     return MFFT_OK;
 }
 
+// Plan creation is the set-up of mfft_exec_noalloc too: the address of its workspace array
+// on the current device is resolved here, so an exec under graph capture or in a serving
+// loop finds it cached (a failure is reported by that exec, not here).
+static void resolve_noalloc_scratch()
+{
+    const std::string keep = g_err;
+    void *ws = nullptr;
+    (void)noalloc_scratch(&ws);
+    g_err = keep;
+}
+
 // split_hint = MFFT_SPLIT_TUNED: take split and chunk from the autotuner's cache.
 static mfft_desc resolve_tuned(const mfft_desc *d, int nx, int ny, int64_t batch)
 {
@@ -844,6 +855,7 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
         g_err = msg;
         return rc;
     }
+    resolve_noalloc_scratch();
     *plan = p;
     return MFFT_OK;
 }
@@ -900,6 +912,7 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
         g_err = msg;
         return rc;
     }
+    resolve_noalloc_scratch();
     *plan = p;
     return MFFT_OK;
 }

@@ -163,6 +163,18 @@ int mfft_batcher_destroy(mfft_batcher b);
  * one plan can then serve several streams at once, each with its own workspace. */
 int mfft_exec_ex(mfft_plan plan, const void *in, void *out, int64_t batch, void *workspace, void *stream);
 
+/* As mfft_exec_ex with no workspace argument and no allocation: the plan's intermediates go
+ * to a device array the library holds for this entry point, MFFT_NOALLOC_SCRATCH_BYTES per
+ * GPU, reserved when the library's code is loaded on the GPU and kept until the process
+ * ends. No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The plan is created beforehand (creation allocates its tables); a plan whose
+ * mfft_info.workspace_bytes exceeds the array is refused with MFFT_ERR_NOMEM (create it
+ * with workspace_limit <= MFFT_NOALLOC_SCRATCH_BYTES). All calls on one GPU share its array:
+ * calls that overlap on one GPU race, so callers order them (one stream, or events). */
+#define MFFT_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int mfft_exec_noalloc(mfft_plan plan, const void *in, void *out, int64_t batch, void *stream);
+
 /* hipGraph-captured exec: the plan's launches for fixed (in, out) pointers are captured
  * once and replayed with one graph launch (no per-launch host work). The graph holds no
  * memset nodes. Destroy before the plan. */
