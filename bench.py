@@ -1,33 +1,39 @@
 """Benchmark of the matrix-core FFT hot path on one MI355X.
 
-Default workload (the metric's configuration): batched 1D complex FFT, N=65536 (radix
+Headline workload (the metric's configuration): batched 1D complex FFT, N=65536 (radix
 16^4, two 256-point passes), batch=256, fp16 interleaved input and output, split-fp16
 (hi+lo) matrix-core stages with fp32 accumulation and per-stage 1/16 scaling, fp16
-intermediates between the two passes (the fp32-intermediate variant is reported beside
-it). One step = one exec of the whole batch with the input already resident in HBM.
+intermediates between the two passes. One step = one exec of the whole batch with the
+input already resident in HBM; `value` is GFLOP/s = 5 N log2(N) batch / t over exactly
+--steps execs after --warmup untimed ones.
 
-Printed (one JSON line): GFLOP/s = 5*N*log2(N)*batch / t, rel-L2 and rel-max vs the FP64
-oracle on a sample of the batch, the plain fp32 GPU FFT (this library's fp32 VALU path)
-and the fp16 / bf16 variants on the same sizes, torch.fft (rocFFT, fp32) as a labelled
-external data point, the roofline of the dominant kernel from HIP events on the exec
-stream, and the FP64 CPU baseline on the host cores.
-
-Serving (the "serve" object of the line): the continuous-batching request server on the
-mixed request workload -- Poisson arrivals at 2000 requests/s (seed 42), 60% 1D FFT,
-25% 2D FFT, 15% conv2d against 32 resident filters, 64 GB HBM budget, split fp16 --
-replayed in real time: p50/p99 latency and throughput, the same trace through the
-server's plain mode (fp32, one request at a time), saturation throughput of both and of
-torch.fft (rocFFT) called once per request, and the highest arrival rate each sustains
-within one p99 bound. --config c5 runs the full 30 s trace; the default line carries a
-shorter one (--serve-seconds).
-
---config c4 times the FFT 'same' convolution (2048x2048 x 16, 31x31 filter, pad 2176).
+The one JSON line also carries (all measured in the same run):
+  * rel-L2 / rel-max against the FP64 oracle; the fp32 (plain GPU FFT of this library,
+    the ratio's denominator), fp16, bf16 and fp32-intermediate variants, torch.fft
+    (rocFFT) as a labelled external point;
+  * `roofline` of the dominant kernel: algorithmic bytes per launch over its average
+    launch time, the kernel's own window (HIP events the runtime records around the
+    launch) inside whole execs run pass by pass, 200 of them after 20 warm-up execs
+    whatever --steps is; `pass_check` compares the windows' sum with a whole exec;
+  * `cpu_baseline`: the FP64 oracle in the reference's CPU harness shape on the host;
+  * `reference_api`: the reference's own entry points on the same shape -- tcfftExec
+    (digit-reversed input, in place; tcfft_half.cu:1177-1233) against the natural-order
+    exec of the same precision, and the setup/doit harness timed as main.cpp:200-223
+    does;
+  * `configs`: the other workloads, compact -- c1 (1D 256 x 4096, the reference's own
+    plan), c3 (2D 1024^2 x 64 with 10% of entries x100, forward + inverse), c4 (FFT
+    'same' conv2d 2048^2 x 16, 31x31 filter, pad 2176) -- each with value, roofline,
+    CPU baseline, accuracy and the ratio to plain fp32;
+  * `serve`: the continuous-batching request server on the c5 trace (Poisson 2000 req/s
+    for --serve-seconds, default the full 30 s): p50/p99, throughput, the same trace in
+    plain per-request fp32 mode, saturation and the equal-p99 ratio.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -40,14 +46,19 @@ sys.path.insert(0, ROOT)
 METRIC = "FFT GFLOP/s (5NlogN) at rel-L2≤1e-3 vs plain FP32 GPU FFT; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 TUNE = True
-TIMING = ""
-PASS_TIMING = ""
+PASS_REPS = 200   # whole execs per pass-window measurement (the roofline's launch time)
+PASS_WARM = 20    # execs run pass by pass before it
+# The device's clocks ramp up over the first ~50 ms of continuous work (65536 x 256: pass 0
+# runs 34 us at first, 28 us after that) and fall back while the host waits. Every timed
+# region is therefore preceded, with no host synchronisation in between, by this much
+# back-to-back device time of the same work (plus the stated warm-up steps).
+SETTLE_MS = 60.0
 
 CONFIGS = {
     # name: (shape, batch, description)
     "c2": ((65536,), 256, "batched 1D complex FFT N=65536 (16^4) x 256, fp16 I/O"),
     "c1": ((256,), 4096, "batched 1D complex FFT N=256 x 4096, fp16 I/O"),
-    "c3": ((1024, 1024), 64, "batched 2D complex FFT 1024x1024 x 64, fp16 I/O"),
+    "c3": ((1024, 1024), 64, "batched 2D complex FFT 1024x1024 x 64, 10% of entries x100, fp16 I/O"),
 }
 
 
@@ -59,8 +70,6 @@ def flops(shape, batch):
 def make_input(shape, batch, seed=42, spikes=False):
     """Interleaved re/im uniform[-1,1] (SplitMix64), rounded to fp16. spikes: 10% of the
     complex entries (mask from the next seed) scaled x100, to stress dynamic range."""
-    import numpy as np
-
     import oracle
 
     n = math.prod(shape)
@@ -71,164 +80,212 @@ def make_input(shape, batch, seed=42, spikes=False):
     return oracle.round_fp16(v).reshape(batch, *shape, 2)
 
 
-def round_trip(shape, batch, xh, dev, sample):
-    """Forward (per-stage 1/radix) then inverse (unscaled) split-fp16 transforms with fp16
-    intermediates on `sample` items: (rel_l2, rel_max) of the round trip against the input."""
-    import numpy as np
-    import torch
-
-    import fft_demo_amd as F
-    import oracle
-
-    fwd = F.Plan(shape, sample, precision="split_fp16", mid_fmt="c16")
-    inv = F.Plan(shape, sample, precision="split_fp16", mid_fmt="c16", scale=0, direction=1)
-    x = torch.from_numpy(np.ascontiguousarray(xh[:sample])).to(device=dev, dtype=fwd.in_dtype()).contiguous()
-    back = inv.exec(fwd.exec(x)).float().cpu().numpy().astype(np.float64)
-    torch.cuda.synchronize()
-    l2, mx = oracle.err_l2max(back[..., 0] + 1j * back[..., 1], xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1])
-    fwd.close()
-    inv.close()
-    return l2, mx
+def _lib():
+    return __import__("fft_demo_amd._lib", fromlist=["load"]).load()
 
 
-def time_plan(plan, x, out, steps, warmup, stream):
-    """Time `steps` back-to-back execs between two HIP events on the exec stream; when an
-    exec is shorter than 0.5 ms (the Python wrapper's launch overhead would then leave gaps
-    in the queue) a hipGraph holding the `steps` execs is timed too and the faster is reported.
-    Each pass is then timed (for the roofline): `steps` whole execs run pass by pass
-    (mfft_exec_pass), the pass's launch in each between a start and a stop HIP event that
-    the runtime records around the kernel itself on the exec stream (its execution window,
-    as a kernel trace reports it); the average is the pass's launch time. (A pass launched
-    alone over and over would find the input it rereads in the last-level cache.) Plans that run their batch in
-    several chunks fall back to two events around `steps` back-to-back runs of the pass
-    (one launch per chunk, launch gaps included), divided by the number of launches.
-    Returns (total_ms, [per-pass avg ms per launch])."""
+# ------------------------------------------------------------------------- timing
+def settle(run, stream, ms=SETTLE_MS):
+    """Queue back-to-back calls of run(stream) worth `ms` of device time (estimated from
+    three synchronised calls) and return without waiting for them: launches that follow
+    queue behind them, on a device whose clocks have ramped up."""
     import torch
 
     from fft_demo_amd.hiptime import EventPool
 
-    npass = len(plan.info["passes"])
-    for _ in range(warmup):
-        plan.exec(x, out, stream=stream)
+    ev = EventPool(2)
+    ev.record(0, stream.cuda_stream)
+    for _ in range(3):
+        run(stream)
+    ev.record(1, stream.cuda_stream)
     torch.cuda.synchronize()
-    lib = __import__("fft_demo_amd._lib", fromlist=["load"]).load()
+    per = max(ev.elapsed_ms(0, 1) / 3, 1e-3)
+    ev.close()
+    n = int(ms / per) + 1
+    for _ in range(n):
+        run(stream)
+    return n
+
+
+def time_execs(run, steps, stream, graph_ok=True):
+    """Time `steps` calls of run(stream) between two HIP events on the stream; when a call
+    is shorter than 0.5 ms (host launch overhead of the Python wrapper is then comparable
+    to the GPU time and can leave gaps in the queue) the same `steps` calls captured in one
+    hipGraph are timed too and the faster is reported. Returns (total_ms, how)."""
+    import torch
+
+    from fft_demo_amd.hiptime import EventPool
+
     ends = EventPool(2)
     ends.record(0, stream.cuda_stream)
     for _ in range(steps):
-        plan.exec(x, out, stream=stream)
+        run(stream)
     ends.record(1, stream.cuda_stream)
     torch.cuda.synchronize()
     total = ends.elapsed_ms(0, 1)
-    global TIMING
-    TIMING = "back-to-back execs between two HIP events"
-    if total / steps < 0.5:
-        # short execs (host launch overhead of the Python wrapper is comparable to the GPU
-        # time): also capture the steps in one hipGraph and time its replay; the faster of
-        # the two ways of running the same execs is reported (graph nodes carry a launch
-        # latency of their own, so neither wins everywhere)
+    how = "back-to-back execs between two HIP events"
+    if graph_ok and total / steps < 0.5:
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             cs = torch.cuda.current_stream()  # the capture stream
             for _ in range(steps):
-                plan.exec(x, out, stream=cs)
+                run(cs)
         g.replay()
         torch.cuda.synchronize()
+        for _ in range(int(SETTLE_MS / max(total, 1e-3)) + 1):  # settle on replays, then time one
+            g.replay()
         ends.record(0, stream.cuda_stream)
         g.replay()
         ends.record(1, stream.cuda_stream)
         torch.cuda.synchronize()
         tg = ends.elapsed_ms(0, 1)
         if tg < total:
-            total = tg
-            TIMING = "hipGraph replay of the steps (captured execs), between two HIP events"
+            total, how = tg, "hipGraph replay of the captured execs, between two HIP events"
         del g
     ends.close()
-    global PASS_TIMING
-    per_pass = []
+    return total, how
+
+
+def pass_windows(plan, x, out, stream):
+    """Average launch time of every pass of the plan. Multi-pass plans: measured inside
+    whole execs run pass by pass -- after a settle phase and PASS_WARM such execs,
+    PASS_REPS execs per pass with that pass's kernel bracketed by start/stop HIP events the
+    runtime records around the launch itself (hipExtLaunchKernel), its execution window as
+    a kernel trace reports it; everything enqueued back to back, one host wait at the end.
+    A plan that runs its batch in several chunks: two events around PASS_REPS runs of the
+    pass (one launch per chunk, gaps included) divided by the launches. A one-pass plan:
+    its exec time, over a hipGraph of PASS_REPS execs when that is faster than launching
+    them one by one (a Python loop cannot keep a few-microsecond kernel fed). The check:
+    the windows' sum against a whole exec timed over PASS_REPS execs right after."""
+    import torch
+
+    from fft_demo_amd.hiptime import EventPool
+
+    lib = _lib()
+    npass = len(plan.info["passes"])
     single = plan.info["batch_chunk"] >= plan.batch
     nchunks = -(-plan.batch // plan.info["batch_chunk"])
-    PASS_TIMING = ("kernel window: start/stop HIP events recorded around the pass's launch "
-                   "(hipExtLaunchKernel) inside whole execs run pass by pass" if single else
-                   f"two HIP events around back-to-back runs of the pass ({nchunks} chunk launches each), per launch")
-    pool = EventPool(2 * steps)
-    for k in range(npass):
-        if single:
-            # every exec runs all its passes in order (so the caches hold what they hold in
-            # an exec); only pass k's launch carries the events
-            for i in range(steps):
-                for j in range(npass):
-                    ev = (pool.arr[2 * i], pool.arr[2 * i + 1]) if j == k else (None, None)
-                    rc = lib.mfft_exec_pass(plan._h, j, x.data_ptr(), out.data_ptr(), stream.cuda_stream, *ev)
-                    if rc != 0:
-                        raise RuntimeError(lib.mfft_last_error().decode())
-            torch.cuda.synchronize()
-            per_pass.append(sum(pool.elapsed_ms(2 * i, 2 * i + 1) for i in range(steps)) / steps)
-        else:
-            pool.record(0, stream.cuda_stream)
-            for _ in range(steps):
-                rc = lib.mfft_exec_pass(plan._h, k, x.data_ptr(), out.data_ptr(), stream.cuda_stream, None, None)
-                if rc != 0:
-                    raise RuntimeError(lib.mfft_last_error().decode())
-            pool.record(1, stream.cuda_stream)
-            torch.cuda.synchronize()
-            per_pass.append(pool.elapsed_ms(0, 1) / steps / nchunks)
-    pool.close()
-    # leave `out` holding a full exec's result
-    plan.exec(x, out, stream=stream)
+    s = stream.cuda_stream
+
+    def run_pass(j, ev=(None, None)):
+        rc = lib.mfft_exec_pass(plan._h, j, x.data_ptr(), out.data_ptr(), s, *ev)
+        if rc != 0:
+            raise RuntimeError(lib.mfft_last_error().decode())
+
+    def whole(st):
+        plan.exec(x, out, stream=st)
+
+    if npass == 1:
+        settle(whole, stream)
+        t, how = time_execs(whole, PASS_REPS, stream)
+        per_pass = [t / PASS_REPS]
+        exec_ms = per_pass[0]
+        how = f"one-pass plan: its exec time per launch, {how}, {PASS_REPS} execs after {SETTLE_MS:g} ms of settle"
+    else:
+        def pbp(st):
+            for j in range(npass):
+                run_pass(j)
+
+        settle(pbp, stream)
+        for _ in range(PASS_WARM):
+            pbp(stream)
+        pool = EventPool(2 * PASS_REPS * npass)
+        for k in range(npass):
+            if single:
+                for i in range(PASS_REPS):
+                    e = 2 * (k * PASS_REPS + i)
+                    for j in range(npass):
+                        run_pass(j, (pool.arr[e], pool.arr[e + 1]) if j == k else (None, None))
+            else:
+                pool.record(2 * k * PASS_REPS, s)
+                for _ in range(PASS_REPS):
+                    run_pass(k)
+                pool.record(2 * k * PASS_REPS + 1, s)
+        exec_ms, _ = time_execs(whole, PASS_REPS, stream, graph_ok=False)
+        exec_ms /= PASS_REPS
+        per_pass = []
+        for k in range(npass):
+            if single:
+                per_pass.append(sum(pool.elapsed_ms(2 * (k * PASS_REPS + i), 2 * (k * PASS_REPS + i) + 1)
+                                    for i in range(PASS_REPS)) / PASS_REPS)
+            else:
+                per_pass.append(pool.elapsed_ms(2 * k * PASS_REPS, 2 * k * PASS_REPS + 1) / PASS_REPS / nchunks)
+        pool.close()
+        how = ("kernel window: start/stop HIP events recorded around the pass's launch (hipExtLaunchKernel) inside "
+               f"{PASS_REPS} whole execs run pass by pass, after {SETTLE_MS:g} ms of settle and {PASS_WARM} warm-up "
+               "execs" if single else
+               f"two HIP events around {PASS_REPS} back-to-back runs of the pass ({nchunks} chunk launches each), "
+               "per launch")
+    total = sum(per_pass) * (nchunks if not single else 1)
+    check = {"reps": PASS_REPS, "warmup": PASS_WARM, "settle_ms": SETTLE_MS, "pass_sum_ms": round(total, 5),
+             "exec_ms": round(exec_ms, 5), "ratio": round(total / exec_ms, 4),
+             "within_5pct": abs(total / exec_ms - 1) <= 0.05}
+    plan.exec(x, out, stream=stream)  # leave `out` holding a full exec's result
     torch.cuda.synchronize()
-    return total, per_pass
+    return per_pass, check, how
 
 
 def accuracy(plan, xh, out, sample, direction=-1):
-    import numpy as np
-
     import oracle
 
-    y = out[:sample].float().cpu().numpy().astype(np.float64)
+    y = out[:sample].float().cpu().numpy().astype("float64")
     yc = (y[..., 0] + 1j * y[..., 1]) / plan.out_scale
     xc = xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1]
     ref = oracle.fft(xc, sign=direction) if len(plan.shape) == 1 else oracle.fft2d(xc, sign=direction)
     return oracle.err_l2max(yc, ref)
 
 
-def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt=None):
+def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt=None, passes=True):
+    """Autotune (split point, batch chunk), then time `steps` execs after `warmup`; the
+    per-pass windows when `passes`."""
     import torch
 
     import fft_demo_amd as F
     from fft_demo_amd import tune
 
-    prec = name
-    # matrix-core variants scale 1/radix per stage (fp16/bf16 would overflow on the 2D
-    # workload's x100 entries otherwise); plain fp32 is unscaled
-    scale = 0 if prec == "fp32" else 1
-    # every variant gets the same treatment: autotuned schedule (split point, batch chunk),
-    # unless --no-tune (profiling runs: only the planner's schedule is launched)
-    sched = tune.autotune(shape, batch, precision=prec, mid_fmt=mid_fmt, scale=scale) if TUNE else None
-    plan = F.Plan(shape, batch, precision=prec, split_hint=tune.SPLIT_TUNED if TUNE else 0, mid_fmt=mid_fmt,
+    scale = 0 if name == "fp32" else 1  # matrix-core variants scale 1/radix per stage
+    sched = tune.autotune(shape, batch, precision=name, mid_fmt=mid_fmt, scale=scale) if TUNE else None
+    plan = F.Plan(shape, batch, precision=name, split_hint=tune.SPLIT_TUNED if TUNE else 0, mid_fmt=mid_fmt,
                   scale=scale)
-    dt = plan.in_dtype()
-    x = torch.from_numpy(xh).to(device=dev, dtype=dt).contiguous()
+    x = torch.from_numpy(xh).to(device=dev, dtype=plan.in_dtype()).contiguous()
     out = plan.empty_out(device=dev)
-    total_ms, per_pass = time_plan(plan, x, out, steps, warmup, stream)
-    ms = total_ms / steps
-    l2, mx = accuracy(plan, xh, out, sample)
-    res = {
-        "gflops": flops(shape, batch) / (ms * 1e-3) / 1e9,
-        "ms_per_exec": ms,
-        "rel_l2": l2,
-        "rel_max": mx,
-        "passes": plan.info["passes"],
-        "pass_avg_ms": per_pass,
-        "batch_chunk": plan.info["batch_chunk"],
-        "launches_per_exec": plan.info["launches_per_exec"],
-        "schedule": sched,
-        "timing": TIMING,
-        "pass_timing": PASS_TIMING,
-    }
+    settle(lambda st: plan.exec(x, out, stream=st), stream)
+    for _ in range(warmup):
+        plan.exec(x, out, stream=stream)
+    total, how = time_execs(lambda st: plan.exec(x, out, stream=st), steps, stream)
+    ms = total / steps
+    res = {"gflops": flops(shape, batch) / (ms * 1e-3) / 1e9, "ms_per_exec": ms, "timing": how,
+           "passes": plan.info["passes"], "batch_chunk": plan.info["batch_chunk"],
+           "launches_per_exec": plan.info["launches_per_exec"], "schedule": sched}
+    if passes:
+        res["pass_avg_ms"], res["pass_check"], res["pass_timing"] = pass_windows(plan, x, out, stream)
+    else:
+        plan.exec(x, out, stream=stream)
+        torch.cuda.synchronize()
+    res["rel_l2"], res["rel_max"] = accuracy(plan, xh, out, sample)
     del x, out
     plan.close()
     torch.cuda.synchronize()
     return res
+
+
+def roofline_of(res, key_prefix=""):
+    """Roofline of the pass with the largest average launch time (algorithmic bytes per
+    launch: one read of its input and one write of its output at their formats)."""
+    pp = res["pass_avg_ms"]
+    k = max(range(len(pp)), key=lambda i: pp[i])
+    p = res["passes"][k]
+    algo = p["points"] * (p["in_elem_bytes"] + p["out_elem_bytes"])
+    achieved = algo / (pp[k] * 1e-3) / 1e9
+    tkey = f"{p['kernel']}:R{p['R']}:c{p['in_elem_bytes'] * 4}>c{p['out_elem_bytes'] * 4}"
+    tmeas = profiled_traffic().get(key_prefix + tkey)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tmeas[0] if tmeas else None,
+            "traffic_source": f"profiles/{tmeas[1]} ({key_prefix + tkey})" if tmeas else None,
+            "kernel": f"pass {k}: {p['kernel']} (R={p['R']}, lines/tile={p['lines_per_tile']}, "
+                      f"c{p['in_elem_bytes'] * 4} in / c{p['out_elem_bytes'] * 4} out)",
+            "algorithmic_bytes_per_launch": algo, "avg_launch_ms": round(pp[k], 5),
+            "launch_time": res.get("pass_timing")}
 
 
 def rocfft_point(shape, batch, xh, steps, warmup, dev):
@@ -265,41 +322,32 @@ def cpu_baseline(shape, batch, xh):
     import oracle
 
     n = math.prod(shape)
-    sample = batch if len(shape) == 1 else max(1, batch // 16)
+    sample = min(batch, max(1, (1 << 24) // n)) if len(shape) == 1 else max(1, min(batch, (1 << 22) // n))
     data = np.ascontiguousarray(xh[:sample].reshape(-1))
-    runs = []
-    for _ in range(6):
-        t = time.perf_counter()
-        if len(shape) == 1:
-            oracle.fft_batch_loop(data, n, sample)
-        else:
-            xc = xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1]
-            oracle.fft2d(xc, threads=1)
-        runs.append(time.perf_counter() - t)
-    runs = runs[1:]
-    t1 = sorted(runs)[len(runs) // 2]
-    ncores = os.cpu_count() or 1
     xc = xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1]
-    runs = []
-    for _ in range(6):
-        t = time.perf_counter()
-        if len(shape) == 1:
-            oracle.fft(xc, threads=0)
-        else:
-            oracle.fft2d(xc, threads=0)
-        runs.append(time.perf_counter() - t)
-    runs = runs[1:]
-    ta = sorted(runs)[len(runs) // 2]
+
+    def med(fn):
+        runs = []
+        for _ in range(6):
+            t = time.perf_counter()
+            fn()
+            runs.append(time.perf_counter() - t)
+        return sorted(runs[1:])[2]
+
+    if len(shape) == 1:
+        t1 = med(lambda: oracle.fft_batch_loop(data, n, sample))
+        ta = med(lambda: oracle.fft(xc, threads=0))
+    else:
+        t1 = med(lambda: oracle.fft2d(xc, threads=1))
+        ta = med(lambda: oracle.fft2d(xc, threads=0))
     f = flops(shape, sample)
     fftw = ctypes.util.find_library("fftw3")
     return {
-        "value": f / t1 / 1e9,
-        "unit": "GFLOP/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"{sample} transforms of the workload, FP64 Stockham oracle, single-thread batch loop "
-                  f"(reference harness shape main.cpp:68-80), one warm-up then median of 5",
-        "all_cores": {"value": f / ta / 1e9, "unit": "GFLOP/s", "cores": ncores},
+        "value": f / t1 / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+        "sample": f"{sample} of the {batch} transforms, FP64 Stockham oracle, "
+                  + ("single-thread batch loop (reference harness shape main.cpp:68-80)" if len(shape) == 1 else
+                     "single thread") + ", one warm-up then median of 5",
+        "all_cores": {"value": f / ta / 1e9, "unit": "GFLOP/s", "cores": os.cpu_count() or 1},
         "host_cpu": _cpu_model(),
         "fftw": f"libfftw3 found ({fftw}), not timed" if fftw else "FFTW unavailable (no libfftw3 on the host)",
     }
@@ -341,7 +389,8 @@ def copy_bandwidth(mib=1024, reps=10):
 def profiled_traffic():
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy
     kernel of known size), from the summaries committed under profiles/: {key: bytes}, key =
-    "<kernel family>:R<points per line>:c<in bits>>c<out bits>"."""
+    "<kernel family>:R<points per line>:c<in bits>>c<out bits>" (conv kernels: "conv:" + the
+    launch-trace name)."""
     out = {}
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
@@ -357,11 +406,371 @@ def profiled_traffic():
     return out
 
 
+# ------------------------------------------------------------------------- reference API
+def reference_api(n, batch, xh, steps, warmup, dev):
+    """The reference's own entry points on the workload's shape. tcfftExec: digit-reversed
+    fp16 input in place (gen_rev of the handle's radix list, as the reference's setup()
+    permutes), timed over `steps` execs between HIP events on the null stream it launches
+    on, against mfft_exec of a natural-order plan of the same precision and scaling (fp16
+    matrix-core products, unnormalised, in place) timed the same way. setup/doit: the
+    harness over natural-order FP64 host data, timed as main.cpp:200-223 does (doit(i)
+    for i = 1, 2, 4, ... until one call takes over 4 ms of host wall time; time per exec =
+    that call's time / i)."""
+    import numpy as np
+    import torch
+
+    import fft_demo_amd as F
+    import oracle
+    from fft_demo_amd import _lib as L
+    from fft_demo_amd.hiptime import EventPool
+
+    lib = L.load()
+    null = torch.cuda.default_stream(dev)
+    h = L.TcfftHandle()
+    lib.tcfftCreate(ctypes.byref(h), n, batch)
+    if lib.tcfftGetLastError() != 0:
+        return {"error": lib.mfft_last_error().decode()}
+    radices = list(h.radices[: h.n_radices])
+    rev = oracle.gen_rev(n, radices)
+    buf = torch.from_numpy(np.ascontiguousarray(xh[:, rev])).to(device=dev, dtype=torch.float16)
+    # accuracy of one exec on the permuted input (a sample of the batch against the oracle)
+    one = buf.clone()
+    lib.tcfftExec(h, ctypes.c_void_p(one.data_ptr()))
+    torch.cuda.synchronize()
+    k = min(batch, 8)
+    y = one[:k].double().cpu().numpy()
+    l2, mx = oracle.err_l2max(y[..., 0] + 1j * y[..., 1], oracle.fft(xh[:k, :, 0] + 1j * xh[:k, :, 1]))
+    del one
+
+    def timed(run):
+        settle(lambda st: run(), null)
+        for _ in range(warmup):
+            run()
+        ev = EventPool(2)
+        ev.record(0, null.cuda_stream)
+        for _ in range(steps):
+            run()
+        ev.record(1, null.cuda_stream)
+        torch.cuda.synchronize()
+        ms = ev.elapsed_ms(0, 1) / steps
+        ev.close()
+        return ms
+
+    # repeated in-place execs run on their own output, as the reference's doit() does
+    t_tc = timed(lambda: lib.tcfftExec(h, ctypes.c_void_p(buf.data_ptr())))
+    info_passes = None
+    with F.Plan((n,), batch, precision="fp16", scale=0, in_order=1) as rp:
+        info_passes = [(p["kernel"], p["R"]) for p in rp.info["passes"]]
+    with F.Plan((n,), batch, precision="fp16", scale=0) as nat:
+        nat_passes = [(p["kernel"], p["R"]) for p in nat.info["passes"]]
+        t_nat = timed(lambda: nat.exec_ptr(buf.data_ptr(), buf.data_ptr(), 0))
+    lib.tcfftDestroy(ctypes.byref(h))
+    del buf
+    # setup / doit / finalize on natural-order FP64 host data
+    data = np.ascontiguousarray(xh.reshape(-1).astype(np.float64))
+    lib.setup(data.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, batch)
+    it, run_ms = 1, 0.0
+    while it <= 1 << 20:
+        t0 = time.perf_counter()
+        lib.doit(it)
+        run_ms = (time.perf_counter() - t0) * 1e3
+        if run_ms > 4.0:
+            break
+        it <<= 1
+    doit_ok = lib.tcfftGetLastError() == 0
+    f = flops((n,), batch)
+    return {
+        "shape": [n, batch], "radices": radices,
+        "tcfftExec": {"ms_per_exec": round(t_tc, 5), "gflops": round(f / (t_tc * 1e-3) / 1e9, 1),
+                      "passes": info_passes, "rel_l2": l2, "rel_max": mx,
+                      "accuracy_sample": f"{k} transforms vs FP64 oracle (fp16 matrix-core, unnormalised)"},
+        "natural_order_exec": {"ms_per_exec": round(t_nat, 5), "gflops": round(f / (t_nat * 1e-3) / 1e9, 1),
+                               "passes": nat_passes},
+        "tcfftExec_vs_natural": round(t_nat / t_tc, 4),
+        "doit": {"iter": it, "ms_per_exec": round(run_ms / it, 5), "gflops": round(f / (run_ms / it * 1e-3) / 1e9, 1),
+                 "ok": doit_ok, "timing": "host wall clock around doit(iter) (main.cpp:200-223), iter doubled "
+                                          "until one call exceeds 4 ms"},
+        "precision": "fp16 matrix-core operands, fp32 accumulate, unnormalised (tcfftCreate's default)",
+    }
+
+
+# ------------------------------------------------------------------------- FFT configs
+def fft_config(cfg, steps, warmup, dev, stream, full=True):
+    """One FFT workload: the split-fp16 plan with fp16 intermediates (the headline
+    precision), its roofline and accuracy, the plain fp32 GPU FFT of the same shape and,
+    when `full`, the fp32-intermediate, fp16 and bf16 variants and rocFFT."""
+    shape, batch, desc = CONFIGS[cfg]
+    spikes = cfg == "c3"
+    xh = make_input(shape, batch, spikes=spikes)
+    sample = 4 if len(shape) == 1 else 1
+    comps = {}
+    # comparators first: the headline is timed last, on a warm device
+    comps["fp32"] = run_variant("fp32", shape, batch, xh, steps, warmup, dev, stream, sample, passes=full)
+    if full:
+        comps["split_fp16_c32_intermediates"] = run_variant("split_fp16", shape, batch, xh, steps, warmup, dev, stream,
+                                                            sample, passes=False)
+        for name in ("fp16", "bf16"):
+            comps[name] = run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, passes=False)
+        try:
+            comps["rocfft_fp32_external"] = rocfft_point(shape, batch, xh, steps, warmup, dev)
+        except Exception as e:  # noqa: BLE001
+            comps["rocfft_fp32_external"] = {"error": str(e)}
+    head = run_variant("split_fp16", shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt="c16")
+    return shape, batch, desc, xh, sample, head, comps
+
+
+def round_trip(shape, batch, xh, dev, sample):
+    """Forward (per-stage 1/radix) then inverse (unscaled) split-fp16 transforms with fp16
+    intermediates on `sample` items: (rel_l2, rel_max) of the round trip against the input."""
+    import numpy as np
+    import torch
+
+    import fft_demo_amd as F
+    import oracle
+
+    with F.Plan(shape, sample, precision="split_fp16", mid_fmt="c16") as fwd, \
+            F.Plan(shape, sample, precision="split_fp16", mid_fmt="c16", scale=0, direction=1) as inv:
+        x = torch.from_numpy(np.ascontiguousarray(xh[:sample])).to(device=dev, dtype=fwd.in_dtype()).contiguous()
+        back = inv.exec(fwd.exec(x)).float().cpu().numpy().astype(np.float64)
+        torch.cuda.synchronize()
+    return oracle.err_l2max(back[..., 0] + 1j * back[..., 1], xh[:sample, ..., 0] + 1j * xh[:sample, ..., 1])
+
+
+def compact_fft(cfg, steps, warmup, dev, stream, with_cpu=True, with_ref=False):
+    shape, batch, desc, xh, sample, head, comps = fft_config(cfg, steps, warmup, dev, stream, full=False)
+    out = {"workload": desc, "value": round(head["gflops"], 1), "unit": "GFLOP/s",
+           "ms_per_exec": round(head["ms_per_exec"], 5), "steps": steps, "timing": head["timing"],
+           "precision": "split_fp16 (hi+lo, fp32 acc), c16 intermediates", "rel_l2": head["rel_l2"],
+           "rel_max": head["rel_max"], "accuracy_sample": f"{sample} of {batch} transforms vs FP64 oracle",
+           "passes": [(p["kernel"], p["R"]) for p in head["passes"]], "pass_avg_ms": head["pass_avg_ms"],
+           "pass_check": head["pass_check"], "roofline": roofline_of(head),
+           "plain_fp32": {"gflops": round(comps["fp32"]["gflops"], 1), "ms_per_exec": round(comps["fp32"]["ms_per_exec"], 5),
+                          "rel_l2": comps["fp32"]["rel_l2"]},
+           "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3)}
+    if len(shape) == 2:
+        rl2, rmx = round_trip(shape, batch, xh, dev, sample)
+        out["round_trip"] = {"rel_l2": rl2, "rel_max": rmx,
+                             "sample": f"{sample} transform(s): forward (1/radix per stage) then inverse, vs the input"}
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline(shape, batch, xh)
+    if with_ref and len(shape) == 1:
+        out["reference_api"] = reference_api(shape[0], batch, xh, steps, warmup, dev)
+    return out
+
+
+# ------------------------------------------------------------------------- MFMA use
+MFMA_FLOPS = 16 * 16 * 32 * 2  # one v_mfma_f32_16x16x32_f16
+MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense fp16
+
+
+def mfma_utilisation(passes, pass_ms):
+    """Matrix-core use of the split-fp16 passes. Every radix-16 stage is a complex DFT16
+    on 16 lines at a time: two v_mfma_f32_16x16x32_f16 per real product (the complex
+    product is folded into K = 32, the accumulator holds 128 of the 256 complex outputs),
+    three products (Fl*Xh + Fh*Xl + Fh*Xh), two for a first stage that reads exact fp16
+    input (no Xl). The count per launch agrees with rocprofv3's SQ_INSTS_MFMA
+    (profiles/fft_fast_split_n65536x256_mfma.md)."""
+    out = []
+    for p, ms in zip(passes, pass_ms):
+        if p["generic"] or not p["kernel"].startswith("fast"):
+            continue
+        n16 = sum(1 for r in p["radix"] if r == 16)
+        groups = p["points"] // 256
+        exact_first = p["in_elem_bytes"] == 4  # c16 in (the split path's fp16 buffers)
+        per_group = 2 * (2 if exact_first else 3) + 2 * 3 * (n16 - 1) if n16 else 0
+        inst = groups * per_group
+        tf = inst * MFMA_FLOPS / (ms * 1e-3) / 1e12
+        out.append({"kernel": p["kernel"], "R": p["R"], "instructions_per_launch": inst,
+                    "avg_launch_ms": round(ms, 5), "achieved_tflops": round(tf, 1),
+                    "peak_tflops": MFMA_PEAK_TFLOPS, "frac": round(tf / MFMA_PEAK_TFLOPS, 4)})
+    return out
+
+
+# ------------------------------------------------------------------------- conv2d
+def _gauss_filter(K=31):
+    import numpy as np
+
+    import oracle
+
+    c = (K - 1) / 2
+    yy, xx = np.mgrid[0:K, 0:K]
+    g = np.exp(-((yy - c) ** 2 + (xx - c) ** 2) / (2 * 25.0))
+    g /= g.sum()
+    g *= 1 + oracle.uniform(K * K, 43, -0.01, 0.01).reshape(K, K)
+    return g
+
+
+def conv_trace(op, x, out, stream, reps=20):
+    """Per-launch windows of a conv forward from the library's launch trace
+    (mfft_trace_begin/end: an event before every launch and one after the last) over `reps`
+    back-to-back forwards after a settle phase, averaged per launch position:
+    [(kernel, avg ms, bytes read, bytes written)]."""
+    import torch
+
+    from fft_demo_amd import _lib as L
+    from fft_demo_amd.hiptime import EventPool
+
+    lib = L.load()
+    settle(lambda st: op.forward(x, out, stream=st), stream)
+    cap = 64 * reps
+    recs = (L.LaunchRecord * cap)()
+    pool = EventPool(cap + 1)
+    L.check(lib.mfft_trace_begin(pool.arr, recs, cap), "mfft_trace_begin")
+    for _ in range(reps):
+        op.forward(x, out, stream=stream)
+    n = ctypes.c_int()
+    L.check(lib.mfft_trace_end(ctypes.c_void_p(stream.cuda_stream), ctypes.byref(n)), "mfft_trace_end")
+    torch.cuda.synchronize()
+    per = n.value // reps
+    out_rows = []
+    for i in range(per):
+        t = sum(pool.elapsed_ms(r * per + i, r * per + i + 1) for r in range(reps)) / reps
+        out_rows.append((recs[i].kernel.decode(), t, recs[i].bytes_read, recs[i].bytes_written))
+    pool.close()
+    return out_rows
+
+
+def conv_config(steps, warmup, with_cpu=True):
+    """FFT 'same' convolution: 2048x2048 fp32 images U[0,1], batch 16, one 31x31 filter
+    (Gaussian sigma 5 normalised to sum 1, times 1 + U[-0.01, 0.01]), padded to 2176; the
+    filter spectrum resident (fp32). Variants: split-fp16 transforms with fp32 image
+    spectra (the configuration's precision), the same with fp16 image spectra and
+    intermediates (half the bytes, rel-L2 ~5e-4), plain fp32 transforms, rocFFT real
+    transforms (external). Accuracy on rows of two images against the direct FP64
+    convolution."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from fft_demo_amd.conv import Conv2d
+
+    B, H, W, K = 16, 2048, 2048, 31
+    img = oracle.uniform(B * H * W, 42, 0.0, 1.0).astype(np.float32).reshape(B, 1, H, W)
+    g = _gauss_filter(K)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    x = torch.from_numpy(img).to(dev)
+    out = torch.empty((B, 1, H, W), dtype=torch.float32, device=dev)
+    gd = torch.from_numpy(g.astype(np.float32)).to(dev).reshape(1, 1, K, K).contiguous()
+
+    def timed(fn):
+        settle(lambda st: fn(), stream)
+        for _ in range(warmup):
+            fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(steps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / steps
+
+    rows = (0, 48)
+
+    def acc():
+        errs = []
+        for b in (0, B - 1):
+            ref = oracle.conv2d_same(img[b, 0].astype(np.float64), g, rows=rows)
+            got = out[b, 0, rows[0]:rows[1]].double().cpu().numpy()
+            errs.append(oracle.err_l2max(got, ref))
+        return max(r[0] for r in errs), max(r[1] for r in errs)
+
+    res = {}
+    traces = {}
+    for name, prec, spec16 in (("fp32", "fp32", False), ("split_fp16_fp16_spectra", "split_fp16", True),
+                               ("split_fp16", "split_fp16", False)):
+        op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision=prec, spec_fp16=spec16)
+        op.set_filters(gd)
+        ms = timed(lambda: op.forward(x, out))
+        l2, mx = acc()
+        traces[name] = conv_trace(op, x, out, stream)
+        res[name] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4), "rel_l2": l2,
+                     "rel_max": mx, "pad": list(op.pad), "image_spectra": "fp16" if spec16 else "fp32"}
+        op.close()
+    # labelled external point: torch.fft real transforms (rocFFT), resident filter spectrum
+    p0, p1 = res["split_fp16"]["pad"]
+    fr = torch.zeros(p0, p1, device=dev)
+    fr[:K, :K] = gd[0, 0]
+    H_spec = torch.fft.rfft2(fr)
+    c0 = (K - 1) // 2
+
+    def torch_conv():
+        y = torch.fft.irfft2(torch.fft.rfft2(x[:, 0], s=(p0, p1)) * H_spec, s=(p0, p1))
+        out[:, 0] = y[:, c0:c0 + H, c0:c0 + W]
+
+    ms = timed(torch_conv)
+    l2, mx = acc()
+    res["rocfft_fp32_external"] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4),
+                                   "rel_l2": l2, "rel_max": mx,
+                                   "note": "torch.fft rfft2/irfft2 (rocFFT) with the filter spectrum precomputed"}
+    del H_spec, fr
+
+    def trace_summary(tr, ms_batch):
+        # dominant launch family: the kernel with the largest summed window, its average launch
+        fam = {}
+        for nm, t, rd, wr in tr:
+            f = fam.setdefault(nm, [0, 0.0, 0])
+            f[0] += 1
+            f[1] += t
+            f[2] += rd + wr
+        nm, (cnt, tsum, bsum) = max(fam.items(), key=lambda kv: kv[1][1])
+        avg_ms, bytes_per = tsum / cnt, bsum / cnt
+        achieved = bytes_per / (avg_ms * 1e-3) / 1e9
+        all_bytes = sum(rd + wr for _, _, rd, wr in tr)
+        tmeas = profiled_traffic().get("conv:" + nm)
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tmeas[0] if tmeas else None,
+                "traffic_source": f"profiles/{tmeas[1]}" if tmeas else None,
+                "kernel": f"{nm} ({cnt} launches per forward)", "algorithmic_bytes_per_launch": int(bytes_per),
+                "avg_launch_ms": round(avg_ms, 5),
+                "launch_time": "HIP events before each launch and after the last (library launch trace), "
+                               f"20 back-to-back forwards after {SETTLE_MS:g} ms of settle, averaged per launch",
+                "forward": {"launches": len(tr), "algorithmic_bytes": int(all_bytes),
+                            "achieved_gbs": round(all_bytes / (ms_batch * 1e-3) / 1e9, 1),
+                            "frac": round(all_bytes / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                "launches": [(n_, round(t, 4), int(rd + wr)) for n_, t, rd, wr in tr]}
+
+    head = res["split_fp16"]
+    P = head["pad"][0] * head["pad"][1]
+    outd = {"workload": "FFT 'same' conv2d, 2048x2048 fp32 x 16, 31x31 Gaussian filter, pad 2176, split-fp16 "
+                        "transforms, fp32 image spectra, resident fp32 filter spectrum",
+            "value": head["images_per_s"], "unit": "images/s", "ms_per_batch": head["ms_per_batch"],
+            "steps": steps, "nominal_gflops": round(B * 2 * 5.0 * P * math.log2(P) / (head["ms_per_batch"] * 1e-3) / 1e9, 1),
+            "pad": head["pad"], "rel_l2": head["rel_l2"], "rel_max": head["rel_max"],
+            "accuracy_sample": f"rows {rows[0]}..{rows[1] - 1} of images 0 and {B - 1} vs direct FP64 convolution",
+            "roofline": trace_summary(traces["split_fp16"], head["ms_per_batch"]),
+            "ratio_vs_plain_fp32": round(head["images_per_s"] / res["fp32"]["images_per_s"], 3),
+            "comparators": {"split_fp16_fp16_spectra": dict(res["split_fp16_fp16_spectra"],
+                                                            roofline=trace_summary(traces["split_fp16_fp16_spectra"],
+                                                                                   res["split_fp16_fp16_spectra"]["ms_per_batch"]),
+                                                            ratio_vs_plain_fp32=round(
+                                                                res["split_fp16_fp16_spectra"]["images_per_s"] /
+                                                                res["fp32"]["images_per_s"], 3)),
+                            "fp32": res["fp32"], "rocfft_fp32_external": res["rocfft_fp32_external"]}}
+    if with_cpu:
+        # direct FP64 'same' convolution (961 MAC per pixel) on 64 rows of one image, 1 thread
+        r = (0, 64)
+        runs = []
+        for _ in range(4):
+            t = time.perf_counter()
+            oracle.conv2d_same(img[0, 0].astype(np.float64), g, rows=r, threads=1)
+            runs.append(time.perf_counter() - t)
+        t1 = sorted(runs[1:])[1]
+        outd["cpu_baseline"] = {"value": round((r[1] - r[0]) / H / t1, 4), "unit": "images/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"rows {r[0]}..{r[1] - 1} of one 2048x2048 image, direct FP64 'same' "
+                                          "convolution (the oracle, main.cpp:97-116 shape), one warm-up then "
+                                          "median of 3, scaled to whole images",
+                                "host_cpu": _cpu_model()}
+    return outd
+
+
 # ------------------------------------------------------------------------- serving
 def _rocfft_plain(trace, filters, dev):
     """torch.fft (rocFFT, complex64) once per request, each finished before the next:
     plain per-request FP32 GPU FFT launches. conv2d = irfft2(rfft2(image) * resident
     filter spectrum) over the same padded size as the server's operator."""
+    import numpy as np
     import torch
 
     from fft_demo_amd import serve
@@ -392,9 +801,9 @@ def _rocfft_plain(trace, filters, dev):
             return torch.fft.fft(xin[: n * b].view(b, n))
         if kind == serve.KIND_2D:
             return torch.fft.fft2(xin[: nx * ny * b].view(b, nx, ny))
-        p, H = filt_spec(nx, fid)
+        p, Hs = filt_spec(nx, fid)
         c = (fmax - 1) // 2
-        y = torch.fft.irfft2(torch.fft.rfft2(img[: nx * ny].view(nx, ny), s=(p, p)) * H, s=(p, p))
+        y = torch.fft.irfft2(torch.fft.rfft2(img[: nx * ny].view(nx, ny), s=(p, p)) * Hs, s=(p, p))
         return y[c:c + nx, c:c + ny]
 
     for r in trace:  # every plan (rocFFT's plan cache) and filter spectrum outside the timed region
@@ -408,8 +817,6 @@ def _rocfft_plain(trace, filters, dev):
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t) * 1e3)
     span = time.perf_counter() - t0
-    import numpy as np
-
     return {"requests": len(trace), "req_per_s": len(trace) / span,
             "gflops": sum(serve.flops(r) for r in trace) / span / 1e9,
             "service_p50_ms": float(np.percentile(lat, 50)), "service_p99_ms": float(np.percentile(lat, 99)),
@@ -418,9 +825,8 @@ def _rocfft_plain(trace, filters, dev):
 
 def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, log):
     """Highest arrival rate (x1.5 steps from start_rate, then four bisection steps) whose
-    p99 stays within the bound;
-    returns the replay result at that rate. shapes_trace covers the request shapes (plans
-    and operators are built from it before the probes)."""
+    p99 stays within the bound; returns the replay result at that rate. shapes_trace covers
+    the request shapes (plans and operators are built from it before the probes)."""
     from fft_demo_amd import serve
 
     best = None
@@ -429,6 +835,7 @@ def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, 
     serve.register_filters(S, filters)
     ar = serve.Arena(S, shapes_trace, 16 << 30)
     serve.warm(S, shapes_trace, ar)
+
     def probe(rate):
         r = serve.replay(S, serve.make_trace(rate, seconds, 43), ar)
         log.append({"rate": round(rate, 1), "p99_ms": round(r["p99_ms"], 3), "req_per_s": round(r["req_per_s"], 1),
@@ -437,6 +844,7 @@ def _sustained(make, filters, shapes_trace, target_p99_ms, seconds, start_rate, 
 
     try:
         fail = None
+        lo = rate
         for _ in range(10):
             r, ok = probe(rate)
             if not ok:
@@ -489,21 +897,23 @@ def serve_section(seconds, sat_requests=2000, p99_bound_ms=2.0):
                      "saturation_gflops": round(sat["gflops"], 1),
                      "launches": st["launches"], "coalesced": st["coalesced"], "paired": st["paired"],
                      "static_bytes": st["static_bytes"], "peak_bytes": st["peak_bytes"]}
+        print(f"[bench] serve {name}: p50 {r['p50_ms']:.3f} ms p99 {r['p99_ms']:.3f} ms", file=sys.stderr, flush=True)
     try:
         out["plain_rocfft_external"] = _rocfft_plain(trace[:sat_requests // 2], filters, torch.device("cuda:0"))
     except Exception as e:  # noqa: BLE001
         out["plain_rocfft_external"] = {"error": str(e)}
     torch.cuda.empty_cache()
     # equal p99: the highest arrival rate each mode sustains with p99 within one bound
-    bound = p99_bound_ms
-    logs = {}
-    sus = {}
+    logs, sus = {}, {}
     for name, kw in modes.items():
         logs[name] = []
-        b = _sustained(lambda kw=kw: serve.Server(**kw), filters, trace, bound, min(seconds, 2.0), 2000.0, logs[name])
+        b = _sustained(lambda kw=kw: serve.Server(**kw), filters, trace, p99_bound_ms, 2.0, 2000.0, logs[name])
         sus[name] = round(b["req_per_s"], 1) if b else None
         torch.cuda.empty_cache()
-    out["equal_p99"] = {"p99_bound_ms": bound, "sustained_req_per_s": sus, "probes": logs,
+        print(f"[bench] serve {name}: sustained {sus[name]} req/s at p99 <= {p99_bound_ms} ms", file=sys.stderr,
+              flush=True)
+    out["equal_p99"] = {"p99_bound_ms": p99_bound_ms, "probe_seconds": 2.0, "sustained_req_per_s": sus,
+                        "probes": logs,
                         "ratio": round(sus["split"] / sus["plain_fp32"], 3) if sus["split"] and sus["plain_fp32"]
                         else None}
     out["saturation_ratio_vs_plain_fp32"] = round(out["split"]["saturation_req_per_s"] /
@@ -514,133 +924,79 @@ def serve_section(seconds, sat_requests=2000, p99_bound_ms=2.0):
     return out
 
 
-# ------------------------------------------------------------------------- MFMA use
-MFMA_FLOPS = 16 * 16 * 32 * 2  # one v_mfma_f32_16x16x32_f16
-MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense fp16
-
-
-def mfma_utilisation(passes, pass_ms):
-    """Matrix-core use of the split-fp16 passes. Every radix-16 stage is a complex DFT16
-    on 16 lines at a time: two v_mfma_f32_16x16x32_f16 per real product (the complex
-    product is folded into K = 32, the accumulator holds 128 of the 256 complex outputs),
-    three products (Fl*Xh + Fh*Xl + Fh*Xh), two for a first stage that reads exact fp16
-    input (no Xl). The count per launch agrees with rocprofv3's SQ_INSTS_MFMA
-    (profiles/fft_fast_split_n65536x256_mfma.md)."""
-    out = []
-    for p, ms in zip(passes, pass_ms):
-        if p["generic"] or not p["kernel"].startswith("fast"):
-            continue
-        n16 = sum(1 for r in p["radix"] if r == 16)
-        groups = p["points"] // 256
-        exact_first = p["in_elem_bytes"] == 4  # c16 in (the split path's fp16 buffers)
-        per_group = 2 * (2 if exact_first else 3) + 2 * 3 * (n16 - 1) if n16 else 0
-        inst = groups * per_group
-        tf = inst * MFMA_FLOPS / (ms * 1e-3) / 1e12
-        out.append({"kernel": p["kernel"], "R": p["R"], "instructions_per_launch": inst,
-                    "avg_launch_ms": round(ms, 5), "achieved_tflops": round(tf, 1),
-                    "peak_tflops": MFMA_PEAK_TFLOPS, "frac": round(tf / MFMA_PEAK_TFLOPS, 4)})
-    return out
-
-
-# ------------------------------------------------------------------------- conv2d
-def conv_section(steps, warmup):
-    """FFT 'same' convolution: 2048x2048 fp32 images U[0,1], batch 16, one 31x31 filter
-    (Gaussian sigma 5 normalised to sum 1, times 1 + U[-0.01, 0.01]), padded to 2176;
-    split-fp16 transforms with fp16 image spectra and intermediates and the fp32 filter
-    spectrum resident (comparators: split fp16 with fp32 spectra, plain fp32). Accuracy on
-    sampled rows of two images against the direct FP64 convolution."""
-    import numpy as np
-    import torch
-
-    import oracle
-    from fft_demo_amd.conv import Conv2d
-
-    B, H, W, K = 16, 2048, 2048, 31
-    img = oracle.uniform(B * H * W, 42, 0.0, 1.0).astype(np.float32).reshape(B, 1, H, W)
-    c = (K - 1) / 2
-    yy, xx = np.mgrid[0:K, 0:K]
-    g = np.exp(-((yy - c) ** 2 + (xx - c) ** 2) / (2 * 25.0))
-    g /= g.sum()
-    g *= 1 + oracle.uniform(K * K, 43, -0.01, 0.01).reshape(K, K)
-    dev = torch.device("cuda:0")
-    x = torch.from_numpy(img).to(dev)
-    out = torch.empty((B, 1, H, W), dtype=torch.float32, device=dev)
-    gd = torch.from_numpy(g.astype(np.float32)).to(dev).contiguous()
-
-    def timed(fn):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record()
-        for _ in range(steps):
-            fn()
-        e.record()
-        torch.cuda.synchronize()
-        return s.elapsed_time(e) / steps
-
-    def accuracy():
-        errs = []
-        for b in (0, B - 1):
-            ref = oracle.conv2d_same(img[b, 0].astype(np.float64), g, rows=rows)
-            got = out[b, 0, rows[0]:rows[1]].double().cpu().numpy()
-            errs.append((float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
-                         float(np.abs(got - ref).max() / np.abs(ref).max())))
-        return max(r[0] for r in errs), max(r[1] for r in errs)
-
-    rows = (0, 48)
-    res = {}
-    for name, prec, spec16 in (("split_fp16", "split_fp16", True), ("split_fp16_fp32_spectra", "split_fp16", False),
-                               ("fp32", "fp32", False)):
-        op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision=prec, spec_fp16=spec16)
-        op.set_filters(gd)
-        ms = timed(lambda: op.forward(x, out))
-        l2, mx = accuracy()
-        res[name] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4), "rel_l2": l2,
-                     "rel_max": mx, "pad": list(op.pad), "spectra": "fp16" if spec16 else "fp32"}
-        P = op.pad[0] * op.pad[1]
-        op.close()
-    # labelled external point: torch.fft real transforms (rocFFT), resident filter spectrum
-    p0, p1 = res["split_fp16"]["pad"]
-    fr = torch.zeros(p0, p1, device=dev)
-    fr[:K, :K] = gd
-    H_spec = torch.fft.rfft2(fr)
-    c0 = (K - 1) // 2
-
-    def torch_conv():
-        y = torch.fft.irfft2(torch.fft.rfft2(x[:, 0], s=(p0, p1)) * H_spec, s=(p0, p1))
-        out[:, 0] = y[:, c0:c0 + H, c0:c0 + W]
-
-    ms = timed(torch_conv)
-    l2, mx = accuracy()
-    res["rocfft_fp32_external"] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4),
-                                   "rel_l2": l2, "rel_max": mx,
-                                   "note": "torch.fft rfft2/irfft2 (rocFFT) with the filter spectrum precomputed"}
-    head = res["split_fp16"]
-    ms = head["ms_per_batch"]
-    return {"workload": "FFT 'same' conv2d, 2048x2048 fp32 x 16, 31x31 Gaussian filter, pad 2176, split fp16, "
-                        "fp16 spectra",
-            "images_per_s": head["images_per_s"], "ms_per_batch": ms,
-            "nominal_gflops": round(B * 2 * 5.0 * P * math.log2(P) / (ms * 1e-3) / 1e9, 1),
-            "pad": head["pad"], "rel_l2": head["rel_l2"], "rel_max": head["rel_max"],
-            "accuracy_sample": f"rows {rows[0]}..{rows[1] - 1} of images 0 and {B - 1} vs direct FP64 convolution",
-            "io_gbs": round(B * H * W * 4 * 2 / (ms * 1e-3) / 1e9, 1),
-            "comparators": {"split_fp16_fp32_spectra": res["split_fp16_fp32_spectra"], "fp32": res["fp32"],
-                            "rocfft_fp32_external": res["rocfft_fp32_external"]}}
+# ------------------------------------------------------------------------- main
+def headline_line(cfg, args, dev, stream):
+    shape, batch, desc, xh, sample, head, comps = fft_config(cfg, args.steps, args.warmup, dev, stream,
+                                                             full=not args.no_compare)
+    pp = head["pass_avg_ms"]
+    roofline = roofline_of(head)
+    roofline["copy_gbs"] = copy_bandwidth()
+    roofline["frac_of_copy"] = round(roofline["achieved"] / roofline["copy_gbs"], 4)
+    n = math.prod(shape)
+    comps_out = {}
+    for k, v in comps.items():
+        v = dict(v)
+        v.pop("passes", None)
+        comps_out[k] = v
+    spikes = cfg == "c3"
+    line = {
+        "metric": METRIC,
+        "value": round(head["gflops"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_exec"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16",
+        "data": "synthetic: re/im uniform[-1,1] from SplitMix64 seed 42" +
+                (", 10% of entries x100 (mask seed 43)" if spikes else "") + ", rounded to fp16",
+        "config": {"workload": desc, "shape": list(shape), "batch": batch, "precision": "split_fp16 (hi+lo, fp32 acc)",
+                   "io": "c16 in / c16 out", "intermediates": "c16 between passes",
+                   "scale": "1/16 per radix-16 stage (out = DFT/N)"},
+        "rel_l2": head["rel_l2"],
+        "rel_max": head["rel_max"],
+        "accuracy_sample": f"{sample} of {batch} transforms vs FP64 oracle",
+        "whole_exec_hbm_gbs": round(batch * n * 8 / (head["ms_per_exec"] * 1e-3) / 1e9, 1),
+        "passes": head["passes"],
+        "pass_avg_ms": [round(v, 5) for v in pp],
+        "pass_timing": head["pass_timing"],
+        "pass_check": head["pass_check"],
+        "batch_chunk": head["batch_chunk"],
+        "schedule": head["schedule"],
+        "timing": head["timing"],
+        "settle": f"{SETTLE_MS:g} ms of back-to-back execs of the same plan queued before the warm-up steps "
+                  "(the device's clocks ramp up over ~50 ms of load); no host wait between them and the timed region",
+        "comparators": comps_out,
+        "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3),
+        "roofline": roofline,
+        "mfma": mfma_utilisation(head["passes"], pp),
+    }
+    if len(shape) == 2:
+        rl2, rmx = round_trip(shape, batch, xh, dev, sample)
+        line["round_trip"] = {"rel_l2": rl2, "rel_max": rmx,
+                              "sample": f"{sample} transform(s): forward (1/radix per stage) then inverse, vs the input"}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(shape, batch, xh)
+    if len(shape) == 1 and not args.no_compare:
+        line["reference_api"] = reference_api(shape[0], batch, xh, max(args.steps, 200), args.warmup, dev)
+    return line
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 200 execs of 65536 x 256 take ~11 ms: long enough for the clocks and caches to
-    # settle (20 read a few percent slow, each pass's launches included); the 256 x 4096
-    # workload (~4 us an exec) times 1000 execs in one graph by default
+    # 200 execs of 65536 x 256 take ~11 ms: long enough for the clocks and caches to settle;
+    # the 256 x 4096 workload (~4 us an exec) times 1000 execs in one graph by default
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; 1000 for c1)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4", "c5"])
-    ap.add_argument("--serve-seconds", type=float, default=4.0, help="trace length of the serving run (c5: 30)")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="trace length of the serving run")
     ap.add_argument("--serve-p99-ms", type=float, default=2.0, help="p99 bound of the equal-p99 comparison")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="headline workload only (no c1/c3/c4 objects)")
     ap.add_argument("--no-tune", action="store_true", help="planner's schedule, no autotuning (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the known-size kernel that "
@@ -673,104 +1029,35 @@ def main():
                           "launches": 5}))
         return
     if args.config == "c4":
-        cv = conv_section(args.steps, args.warmup)
-        print(json.dumps({"metric": METRIC, "value": cv["images_per_s"], "unit": "images/s", "n_gpus": 1,
+        cv = conv_config(args.steps, args.warmup, with_cpu=not args.no_cpu)
+        print(json.dumps({"metric": METRIC, "value": cv["value"], "unit": "images/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": cv["ms_per_batch"],
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp16",
-                          "data": "synthetic", "config": {"workload": cv["workload"]}, "conv": cv}))
+                          "data": "synthetic", "config": {"workload": cv["workload"]}, "roofline": cv["roofline"],
+                          "cpu_baseline": cv.get("cpu_baseline"), "conv": cv}))
         return
     if args.config == "c5":
-        sv = serve_section(30.0, p99_bound_ms=args.serve_p99_ms)
+        sv = serve_section(args.serve_seconds, p99_bound_ms=args.serve_p99_ms)
         print(json.dumps({"metric": METRIC, "value": sv["split"]["p99_ms"], "unit": "ms (p99)", "n_gpus": 1,
                           "steps": sv["requests"], "warmup": 0, "ms_per_step": None, "higher_is_better": False,
                           "scaling": "weak", "vs_baseline": None, "dtype": "fp16", "data": "synthetic",
                           "config": {"workload": sv["workload"]}, "serve": sv}))
         return
-    shape, batch, desc = CONFIGS[args.config]
-    spikes = args.config == "c3"  # the 2D workload carries 10% entries x100
-    xh = make_input(shape, batch, spikes=spikes)
-    sample = 4 if len(shape) == 1 else 1
-
-    # headline: split fp16 with fp16 intermediates between passes (one rounding per pass
-    # boundary; rel-L2 stays within 1e-3); the fp32-intermediate variant is a comparator
-    head = run_variant("split_fp16", shape, batch, xh, args.steps, args.warmup, dev, stream, sample, mid_fmt="c16")
-    comps = {}
-    if not args.no_compare:
-        comps["split_fp16_c32_intermediates"] = run_variant("split_fp16", shape, batch, xh, args.steps, args.warmup,
-                                                            dev, stream, sample)
-        for name in ("fp32", "fp16", "bf16"):
-            comps[name] = run_variant(name, shape, batch, xh, args.steps, args.warmup, dev, stream, sample)
-        try:
-            comps["rocfft_fp32_external"] = rocfft_point(shape, batch, xh, args.steps, args.warmup, dev)
-        except Exception as e:  # noqa: BLE001
-            comps["rocfft_fp32_external"] = {"error": str(e)}
-    for v in comps.values():
-        v.pop("passes", None)
-
-    # roofline of the dominant kernel (the pass with the largest average launch time)
-    pp = head["pass_avg_ms"]
-    k = max(range(len(pp)), key=lambda i: pp[i])
-    pinfo = head["passes"][k]
-    algo_bytes = pinfo["points"] * (pinfo["in_elem_bytes"] + pinfo["out_elem_bytes"])
-    achieved = algo_bytes / (pp[k] * 1e-3) / 1e9
-    tkey = f"{pinfo['kernel']}:R{pinfo['R']}:c{pinfo['in_elem_bytes'] * 4}>c{pinfo['out_elem_bytes'] * 4}"
-    tmeas = profiled_traffic().get(tkey)
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": tmeas[0] if tmeas else None,
-        "traffic_source": f"profiles/{tmeas[1]} ({tkey})" if tmeas else None,
-        "kernel": f"pass {k}: {pinfo['kernel']} (R={pinfo['R']}, lines/tile={pinfo['lines_per_tile']}, "
-                  f"c{pinfo['in_elem_bytes'] * 4} in / c{pinfo['out_elem_bytes'] * 4} out)",
-        "algorithmic_bytes_per_launch": algo_bytes,
-        "avg_launch_ms": round(pp[k], 5),
-    }
-    roofline["copy_gbs"] = copy_bandwidth()
-    roofline["frac_of_copy"] = round(achieved / roofline["copy_gbs"], 4)
-    mfma = mfma_utilisation(head["passes"], pp)
-    n = math.prod(shape)
-    whole_bytes = batch * n * 8  # fp16 complex in + out
-    line = {
-        "metric": METRIC,
-        "value": round(head["gflops"], 2),
-        "unit": "GFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(head["ms_per_exec"], 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp16",
-        "data": "synthetic: re/im uniform[-1,1] from SplitMix64 seed 42" +
-                (", 10% of entries x100 (mask seed 43)" if spikes else "") + ", rounded to fp16",
-        "config": {"workload": desc, "shape": list(shape), "batch": batch, "precision": "split_fp16 (hi+lo, fp32 acc)",
-                   "io": "c16 in / c16 out", "intermediates": "c16 between passes",
-                   "scale": "1/16 per radix-16 stage (out = DFT/N)"},
-        "rel_l2": head["rel_l2"],
-        "rel_max": head["rel_max"],
-        "accuracy_sample": f"{sample} of {batch} transforms vs FP64 oracle",
-        "whole_exec_hbm_gbs": round(whole_bytes / (head["ms_per_exec"] * 1e-3) / 1e9, 1),
-        "passes": head["passes"],
-        "pass_avg_ms": [round(v, 5) for v in pp],
-        "pass_timing": head["pass_timing"],
-        "batch_chunk": head["batch_chunk"],
-        "schedule": head["schedule"],
-        "timing": head["timing"],
-        "comparators": comps,
-        "ratio_vs_plain_fp32": round(head["gflops"] / comps["fp32"]["gflops"], 3) if "fp32" in comps else None,
-        "roofline": roofline,
-        "mfma": mfma,
-    }
-    if len(shape) == 2:
-        rl2, rmx = round_trip(shape, batch, xh, dev, sample)
-        line["round_trip"] = {"rel_l2": rl2, "rel_max": rmx,
-                              "sample": f"{sample} transform(s): forward (1/radix per stage) then inverse, vs the input"}
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(shape, batch, xh)
+    line = headline_line(args.config, args, dev, stream)
+    print(f"[bench] {args.config}: {line['value']} GFLOP/s", file=sys.stderr, flush=True)
+    if args.config == "c2" and not args.no_configs:
+        torch.cuda.empty_cache()
+        cfgs = {}
+        cfgs["c1"] = compact_fft("c1", 1000, 20, dev, stream, with_cpu=not args.no_cpu, with_ref=True)
+        print(f"[bench] c1: {cfgs['c1']['value']} GFLOP/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        cfgs["c3"] = compact_fft("c3", 200, 20, dev, stream, with_cpu=not args.no_cpu)
+        print(f"[bench] c3: {cfgs['c3']['value']} GFLOP/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        cfgs["c4"] = conv_config(20, 3, with_cpu=not args.no_cpu)
+        print(f"[bench] c4: {cfgs['c4']['value']} images/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        line["configs"] = cfgs
     if not args.no_serve:
         torch.cuda.empty_cache()
         line["serve"] = serve_section(args.serve_seconds, p99_bound_ms=args.serve_p99_ms)

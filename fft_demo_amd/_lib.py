@@ -179,6 +179,16 @@ SIGNATURES.update({
 })
 
 
+class LaunchRecord(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_char * 64), ("bytes_read", ctypes.c_int64), ("bytes_written", ctypes.c_int64)]
+
+
+SIGNATURES.update({
+    "mfft_trace_begin": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(LaunchRecord), ctypes.c_int]),
+    "mfft_trace_end": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+})
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("split_hint", ctypes.c_int), ("workspace_limit", ctypes.c_int64), ("ms", ctypes.c_double),
                 ("default_ms", ctypes.c_double), ("candidates", ctypes.c_int)]

@@ -86,6 +86,9 @@ bool good(int n)
     return n == 1 || n == 3 || n == 5 || n == 9 || n == 15 || n == 17;
 }
 
+// bytes of one image or product spectrum element
+long long spec_eb(const mfft_conv2d_s *op) { return op->spec_fmt == MFFT_FMT_C16 ? 4 : 8; }
+
 // One call on the operator: holds its mutex and orders its buffers (H, X, Y, G) after the
 // previous call's when that ran on another stream.
 struct OpCall {
@@ -232,6 +235,8 @@ int mfft_conv2d_set_filters(mfft_conv2d op, const void *filters, int real_fmt, v
     OpCall call(op, s);
     if (call.err != hipSuccess) return cfail(MFFT_ERR_HIP, "stream order: %s", hipGetErrorString(call.err));
     const mfft_conv2d_desc &d = op->d;
+    mfft::trace_launch(s, "conv_pack_filters", (long long)d.filters * d.channels * d.fh * d.fw * (real_fmt == MFFT_REAL_F64 ? 8 : 4),
+                       op->spec_bytes);
     if (mfft::conv_pack_filters(filters, real_fmt == MFFT_REAL_F64, op->H, d.filters * d.channels, d.fh, d.fw, op->Ph,
                                 op->Pw, d.mode == MFFT_CONV_VALID_CORR, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "pack filters");
@@ -262,6 +267,8 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
         f.real_w = d.w;
         rc = mfft::exec_fused(op->fwd_img, op->X, op->X, op->pairs * d.channels, s, f);
     } else {
+        mfft::trace_launch(s, "conv_pack", (long long)d.batch * d.channels * d.h * d.w * (in_fmt == MFFT_REAL_F64 ? 8 : 4),
+                           op->pairs * d.channels * (long long)op->Ph * op->Pw * spec_eb(op));
         if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, op->spec_fmt, d.batch, d.channels, d.h, d.w,
                                   op->Ph, op->Pw, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack");
@@ -278,11 +285,15 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
         f.in_mul_period = P;
         rc = mfft::exec_fused(op->inv, op->X, op->Y, op->pairs, s, f);
     } else {
+        mfft::trace_launch(s, "conv_pointwise", op->pairs * d.channels * P * spec_eb(op) + (long long)nf * d.channels * P * 8,
+                           op->pairs * nf * P * spec_eb(op));
         if (mfft::conv_pointwise(op->X, H, op->Y, op->spec_fmt, op->pairs, d.channels, nf, P, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pointwise");
         rc = mfft_exec_ex(op->inv, op->Y, op->Y, op->pairs * nf, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
+    mfft::trace_launch(s, "conv_crop", op->pairs * nf * (long long)op->oh * op->ow * spec_eb(op),
+                       (long long)d.batch * nf * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
     if (mfft::conv_crop(op->Y, op->spec_fmt, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
                         op->Pw, op->oy, op->ox, 1.0f, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
@@ -326,17 +337,21 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
         f.real_w = d.w;
         rc = mfft::exec_fused(op->fwd_img, op->X, op->X, 1, s, f);
     } else {
+        mfft::trace_launch(s, "conv_pack_two", 2LL * d.h * d.w * (in_fmt == MFFT_REAL_F64 ? 8 : 4), P * spec_eb(op));
         if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, op->spec_fmt, d.h, d.w, op->Ph, op->Pw, s) !=
             hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack");
         rc = mfft_exec_ex(op->fwd_img, op->X, op->X, 1, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
+    mfft::trace_launch(s, "conv_pointwise_two", P * spec_eb(op) + 2 * P * 8, P * spec_eb(op));
     if (mfft::conv_pointwise_two(op->X, op->H + filter0 * P, op->H + filter1 * P, op->Y, op->spec_fmt, op->Ph, op->Pw,
                                  s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "pointwise");
     rc = mfft_exec_ex(op->inv, op->Y, op->Y, 1, nullptr, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
+    mfft::trace_launch(s, "conv_crop_two", (long long)op->oh * op->ow * spec_eb(op),
+                       2LL * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
     if (mfft::conv_crop_two(op->Y, op->spec_fmt, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy,
                             op->ox, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
@@ -393,32 +408,44 @@ int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int 
     const int f64 = real_fmt == MFFT_REAL_F64;
     const long long P = (long long)op->Ph * op->Pw;
     // delta spectra (scaled 1/P) in Y
+    mfft::trace_launch(s, "conv_pack", (long long)d.batch * d.filters * op->oh * op->ow * (f64 ? 8 : 4),
+                       op->pairs * d.filters * P * 8);
     if (mfft::conv_pack_pairs(delta, f64, op->Y, MFFT_FMT_C32, d.batch, d.filters, op->oh, op->ow, op->Ph, op->Pw, s) !=
         hipSuccess)
         return cfail(MFFT_ERR_HIP, "pack delta");
     rc = mfft_exec(op->fwd_delta, op->Y, op->Y, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
     if (grad_w) {
+        mfft::trace_launch(s, "conv_pack", (long long)d.batch * d.channels * d.h * d.w * (f64 ? 8 : 4),
+                           op->pairs * d.channels * P * 8);
         if (mfft::conv_pack_pairs(in, f64, op->X, MFFT_FMT_C32, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw, s) !=
             hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack input");
         rc = mfft_exec(op->fwd_img, op->X, op->X, s);
         if (rc) return cfail(rc, "%s", mfft_last_error());
+        mfft::trace_launch(s, "conv_pointwise_gw", op->pairs * (d.channels + d.filters) * P * 8,
+                           (long long)d.filters * d.channels * P * 8);
         if (mfft::conv_pointwise_gw(op->X, op->Y, op->G, (int)op->pairs, d.channels, d.filters, op->Ph, op->Pw, s) !=
             hipSuccess)
             return cfail(MFFT_ERR_HIP, "pointwise (filter gradient)");
         rc = mfft_exec(op->inv_fc, op->G, op->G, s);
         if (rc) return cfail(rc, "%s", mfft_last_error());
         // both spectra carry 1/P and the unnormalised inverse one factor P back
+        mfft::trace_launch(s, "conv_crop_real", (long long)d.filters * d.channels * d.fh * d.fw * 8,
+                           (long long)d.filters * d.channels * d.fh * d.fw * (f64 ? 8 : 4));
         if (mfft::conv_crop_real(op->G, grad_w, f64, (long long)d.filters * d.channels, d.fh, d.fw, op->Ph, op->Pw,
                                  (float)P, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "crop (filter gradient)");
     }
     if (grad_in) {
+        mfft::trace_launch(s, "conv_pointwise_gi", op->pairs * d.filters * P * 8 + (long long)d.filters * d.channels * P * 8,
+                           op->pairs * d.channels * P * 8);
         if (mfft::conv_pointwise_gi(op->Y, op->H, op->X, op->pairs, d.channels, d.filters, P, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pointwise (input gradient)");
         rc = mfft_exec(op->inv_pc, op->X, op->X, s);
         if (rc) return cfail(rc, "%s", mfft_last_error());
+        mfft::trace_launch(s, "conv_crop_cyclic", op->pairs * d.channels * (long long)d.h * d.w * 8,
+                           (long long)d.batch * d.channels * d.h * d.w * (f64 ? 8 : 4));
         if (mfft::conv_crop_cyclic(op->X, grad_in, f64, d.batch, d.channels, d.h, d.w, op->Ph, op->Pw,
                                    op->Ph - (d.fh - 1), op->Pw - (d.fw - 1), 1.0f, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "crop (input gradient)");

@@ -28,6 +28,13 @@ int exec_fused(mfft_plan p, const void *in, void *out, int64_t batch, void *stre
 // Set the calling thread's message returned by mfft_last_error().
 void set_last_error(const char *msg);
 
+// Launch trace (mfft_trace_begin/end): every launch site calls this just before its
+// kernel launch on stream s, with the kernel family and its algorithmic bytes. No-op
+// unless the calling thread has a trace active.
+void trace_launch(hipStream_t s, const char *kernel, long long bytes_read, long long bytes_written);
+bool trace_active();
+const char *fmt_name(int fmt);  // "c16", "cbf16", "c32"
+
 // The device address, on the current device, of the array mfft_exec_noalloc uses as its
 // workspace (resolved with hipGetSymbolAddress on first use per device, then cached).
 int noalloc_scratch(void **out);

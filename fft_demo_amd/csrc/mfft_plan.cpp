@@ -680,6 +680,39 @@ static void *buf_ptr(int which, const void *in, void *out, void *work)
     return which == BUF_IN ? const_cast<void *>(in) : which == BUF_OUT ? out : work;
 }
 
+// Launch-trace record of one pass launch: kernel family, line length and formats, and the
+// algorithmic bytes (every element read and written once; a fused real-image input reads
+// the images' pixels, a fused multiplier its table once).
+static void trace_pass(const mfft_plan_s *p, const PassSpec &ps, const TileArgs &a, long long nb, long long b0,
+                       hipStream_t s)
+{
+    const char *fam = ps.small ? "stream"
+                      : ps.fast_kind == KIND_ROWS ? "fast_rows"
+                      : ps.fast_kind == KIND_ROWS_TR ? "fast_rows_tr"
+                      : ps.fast_kind == KIND_COLS ? "fast_cols"
+                      : ps.generic ? "tile_generic" : "tile";
+    const long long pts = (long long)a.c0 * a.c1 * a.c2 * ps.R;
+    long long rd = pts * fmt_bytes(a.in_fmt);
+    char src[16];
+    snprintf(src, sizeof src, "%s", fmt_name(a.in_fmt));
+    if (a.real_src) {
+        // transforms b0 .. b0 + nb - 1: transform t holds channel t % C of images 2k and 2k + 1
+        // (k = t / C; the second from real_src2 when given, absent past the batch)
+        long long planes = 0;
+        for (long long t = b0; t < b0 + nb; ++t) {
+            const long long k = t / std::max(1, a.real_c);
+            planes += (2 * k < a.real_b) + (a.real_src2 ? 1 : (2 * k + 1 < a.real_b));
+        }
+        rd = planes * a.real_h * a.real_w * (a.real_f64 ? 8 : 4);
+        snprintf(src, sizeof src, "%s", a.real_f64 ? "real64" : "real32");
+    }
+    if (a.in_mul) rd += std::min(pts, a.in_mul_period) * 8;
+    char name[64];
+    snprintf(name, sizeof name, "%s R%d %s%s>%s", fam, ps.R, src, a.in_mul ? "*H" : "", fmt_name(a.out_fmt));
+    (void)p;
+    trace_launch(s, name, rd, pts * fmt_bytes(a.out_fmt));
+}
+
 // only >= 0: run that pass alone; kstart/kstop: events the runtime records around that
 // pass's kernel (its own execution window; one launch per exec).
 static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, hipStream_t s, hipEvent_t *ev,
@@ -754,6 +787,7 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         long long ntiles = ((a.c0 + a.T - 1) / a.T) * (long long)a.c1 * a.c2;
         if (ntiles <= 0) continue;
         if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
+        if (trace_active()) trace_pass(p, ps, a, nb, b0, s);
         if (ev) HIP_TRY(hipEventRecord(ev[evi++], s));
         auto launch = [&](dim3 grid, dim3 block) -> hipError_t {
             void *args[] = {&a};
@@ -779,6 +813,11 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
 }
 
 // ------------------------------------------------------------------ C ABI
+namespace mfft {
+void mfft_trace_reset(hipEvent_t *events, mfft_launch_record *rec, int cap);
+hipError_t mfft_trace_stop(int *n, bool *overflow, hipEvent_t *last);
+}  // namespace mfft
+
 extern "C" {
 
 int mfft_desc_default(mfft_desc *d)
@@ -1142,6 +1181,17 @@ int mfft_batcher_exec(mfft_batcher b, const mfft_batch_item *items, int nitems, 
     HIP_TRY(hipMemcpyAsync(sl.d_jobs, sl.h_jobs, sizeof(TileArgs) * nitems, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(sl.d_first, sl.h_first, sizeof(long long) * (nitems + 1), hipMemcpyHostToDevice, s));
     int grid = (int)std::min<long long>(b->grid, total);
+    if (trace_active()) {
+        // one record for the counter reset and the work-queue kernel
+        long long rd = 0, wr = 0;
+        for (int i = 0; i < nitems; ++i) {
+            rd += items[i].count * items[i].n * (long long)fmt_bytes(b->d.in_fmt);
+            wr += items[i].count * items[i].n * (long long)fmt_bytes(b->d.out_fmt);
+        }
+        char name[64];
+        snprintf(name, sizeof name, "queue %d items %s>%s", nitems, fmt_name(b->d.in_fmt), fmt_name(b->d.out_fmt));
+        trace_launch(s, name, rd, wr);
+    }
     HIP_TRY(launch_queue(b->d.precision, sl.d_jobs, sl.d_first, nitems, total, sl.counter, grid, s));
     HIP_TRY(hipEventRecord(sl.done, s));
     sl.used = true;
@@ -1237,8 +1287,35 @@ int mfft_stage_convert(const void *src, int src_fmt, void *dst, int dst_fmt, int
     if (src_fmt < 0 || src_fmt > 3 || dst_fmt < 0 || dst_fmt > 3) return fail(MFFT_ERR_INVALID, "format");
     if (perm_y && (ny <= 0 || n_inner % ny != 0)) return fail(MFFT_ERR_INVALID, "2D staging needs ny | n_inner");
     if (src == dst) return fail(MFFT_ERR_INVALID, "staging is out of place");
+    if (trace_active()) {
+        auto eb = [](int f) -> long long { return f == MFFT_FMT_C64 ? 16 : f == MFFT_FMT_C32 ? 8 : 4; };
+        trace_launch((hipStream_t)stream, "stage_convert", batch * n_inner * eb(src_fmt), batch * n_inner * eb(dst_fmt));
+    }
     HIP_TRY(stage_convert(src, src_fmt, dst, dst_fmt, n_inner, batch, perm_x, nx, perm_y, ny, scale,
                           (hipStream_t)stream));
+    return MFFT_OK;
+}
+
+int mfft_trace_begin(void **events, mfft_launch_record *records, int capacity)
+{
+    if (!events || !records || capacity < 1) return fail(MFFT_ERR_INVALID, "trace: bad argument");
+    if (mfft::trace_active()) return fail(MFFT_ERR_INVALID, "trace: already active on this thread");
+    mfft_trace_reset((hipEvent_t *)events, records, capacity);
+    return MFFT_OK;
+}
+
+int mfft_trace_end(void *stream, int *nlaunches)
+{
+    if (!nlaunches) return fail(MFFT_ERR_INVALID, "trace: null argument");
+    int n = 0;
+    bool overflow = false;
+    hipEvent_t last = nullptr;
+    hipError_t err = mfft_trace_stop(&n, &overflow, &last);
+    *nlaunches = n;
+    if (!last) return fail(MFFT_ERR_INVALID, "trace: not active on this thread");
+    if (err != hipSuccess) return fail(MFFT_ERR_HIP, "trace: event record: %s", hipGetErrorString(err));
+    HIP_TRY(hipEventRecord(last, (hipStream_t)stream));
+    if (overflow) return fail(MFFT_ERR_INVALID, "trace: more launches than the %d records", n);
     return MFFT_OK;
 }
 
@@ -1253,6 +1330,62 @@ namespace mfft {
 void set_last_error(const char *msg) { g_err = msg ? msg : ""; }
 
 void digit_reversal_radices(int n, std::vector<int> &out) { rev_radices(n, out); }
+
+// ------------------------------------------------------------------ launch trace
+namespace {
+struct Trace {
+    hipEvent_t *events = nullptr;
+    mfft_launch_record *rec = nullptr;
+    int cap = 0, n = 0;
+    bool overflow = false;
+    hipError_t err = hipSuccess;
+};
+thread_local Trace t_trace;
+thread_local bool t_trace_on = false;
+}  // namespace
+
+bool trace_active() { return t_trace_on; }
+
+void mfft_trace_reset(hipEvent_t *events, mfft_launch_record *rec, int cap)
+{
+    t_trace = Trace();
+    t_trace.events = events;
+    t_trace.rec = rec;
+    t_trace.cap = cap;
+    t_trace_on = true;
+}
+
+// ends the trace: n launches recorded; *last = the event to record after them (null when no
+// trace was active)
+hipError_t mfft_trace_stop(int *n, bool *overflow, hipEvent_t *last)
+{
+    *last = nullptr;
+    if (!t_trace_on) return hipSuccess;
+    t_trace_on = false;
+    *n = t_trace.n;
+    *overflow = t_trace.overflow;
+    *last = t_trace.events[t_trace.n];
+    return t_trace.err;
+}
+
+const char *fmt_name(int fmt) { return fmt == FMT_C16 ? "c16" : fmt == FMT_CBF16 ? "cbf16" : "c32"; }
+
+void trace_launch(hipStream_t s, const char *kernel, long long bytes_read, long long bytes_written)
+{
+    if (!t_trace_on) return;
+    Trace &t = t_trace;
+    if (t.n >= t.cap) {
+        t.overflow = true;
+        return;
+    }
+    const hipError_t e = hipEventRecord(t.events[t.n], s);
+    if (e != hipSuccess && t.err == hipSuccess) t.err = e;
+    mfft_launch_record &r = t.rec[t.n];
+    snprintf(r.kernel, sizeof r.kernel, "%s", kernel);
+    r.bytes_read = bytes_read;
+    r.bytes_written = bytes_written;
+    ++t.n;
+}
 
 static bool capturing(hipStream_t s)
 {

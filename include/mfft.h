@@ -362,6 +362,25 @@ int mfft_server_get_stats(mfft_server server, mfft_server_stats *stats);
 int mfft_server_destroy(mfft_server server);
 const char *mfft_server_last_error(void);
 
+/* ------------------------------------------------------------------------------------
+ * Launch trace of the calling thread (per-kernel timing of composite calls such as a conv
+ * forward; no counterpart in the reference). Between mfft_trace_begin and mfft_trace_end
+ * every kernel the library launches from this thread first records events[k] (hipEvent_t)
+ * on its stream and fills records[k] with the kernel family and the algorithmic bytes it
+ * reads and writes (each input and output element once); mfft_trace_end records
+ * events[n] on `stream` after the last launch and returns n. Launch k then ran between
+ * events[k] and events[k + 1] when every launch went to one stream. Launches beyond
+ * `capacity` (nevents = capacity + 1 events) are not recorded and make mfft_trace_end
+ * return MFFT_ERR_INVALID.
+ * ------------------------------------------------------------------------------------ */
+typedef struct mfft_launch_record {
+    char kernel[64];        /* kernel family, line length and formats, e.g. "fast_cols R256 c16>c16" */
+    int64_t bytes_read, bytes_written;
+} mfft_launch_record;
+
+int mfft_trace_begin(void **events, mfft_launch_record *records, int capacity);
+int mfft_trace_end(void *stream, int *nlaunches);
+
 /* Message of the calling thread's last error ("" if none). */
 const char *mfft_last_error(void);
 

@@ -57,7 +57,8 @@ def test_struct_layouts_match_headers(tmp_path):
              ("mfft_batch_item", _lib.BatchItem), ("mfft_tune_entry", _lib.TuneEntry),
              ("mfft_server_desc", _lib.ServerDesc), ("mfft_request", _lib.Request),
              ("mfft_completion", _lib.Completion), ("mfft_server_stats", _lib.ServerStats),
-             ("tcfftHandle", _lib.TcfftHandle), ("tcfftHandle_2d", _lib.TcfftHandle2d)]
+             ("tcfftHandle", _lib.TcfftHandle), ("tcfftHandle_2d", _lib.TcfftHandle2d),
+             ("mfft_launch_record", _lib.LaunchRecord)]
     lines = []
     want = []
     for cname, py in pairs:
