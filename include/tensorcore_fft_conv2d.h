@@ -15,7 +15,8 @@
  *   conv2d_forward(double *out)                  .cpp:283-290   host FP64 output
  *   conv2d_forward(in, w, out, S, F, sy, sx, fy, fx, GPU)  .cpp:291-363
  *   fft_2d_c2c / ifft_2d_c2c                     .cpp:79-214   host FP64 complex batches
- *                                                     over num_streams streams
+ *                                                     over num_streams streams, staged
+ *                                                     through pinned host memory
  *   conv2d_backward(double *delta_in)             .cpp:364-442  host FP64 dL/dout in;
  *                                                     delta_for_filter [F][S][fy][fx] and
  *                                                     delta_for_signal [S][sy][sx] out
@@ -32,6 +33,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -67,7 +69,12 @@ class fft_conv2d_half {
     {
         init(signal_, filter_, S_, F_, sy_, sx_, fy_, fx_, gpu);
     }
-    ~fft_conv2d_half() { release(); }
+    ~fft_conv2d_half()
+    {
+        release();
+        for (mfft_plan p : plans_) mfft_plan_destroy(p);
+        c2c_release();
+    }
     fft_conv2d_half(const fft_conv2d_half &) = delete;
     fft_conv2d_half &operator=(const fft_conv2d_half &) = delete;
 
@@ -214,63 +221,104 @@ class fft_conv2d_half {
         if (d_signal_) (void)hipFree(d_signal_);
         if (d_out_) (void)hipFree(d_out_);
         d_signal_ = d_out_ = nullptr;
-        for (mfft_plan p : plans_) mfft_plan_destroy(p);
-        plans_.clear();
     }
 
-    void c2c(const double *input, double *out, int x, int y, int batch, int num_streams, int dir)
+    // fft_2d_c2c state, kept across calls while the shape, split and direction stay the same:
+    // one plan for a stream's block of transforms (external workspace), per-stream device
+    // buffers and workspaces, and pinned host staging (the caller's arrays are pageable, and
+    // a copy from pageable memory does not overlap with other streams' work)
+    struct C2C {
+        int x = 0, y = 0, per = 0, nst = 0, dir = 0;
+        mfft_plan plan = nullptr;
+        std::vector<hipStream_t> st;
+        std::vector<void *> dev64, dev32, ws, pin_in, pin_out;
+    } c2c_;
+
+    void c2c_release()
     {
-        if (num_streams < 1) num_streams = 1;
-        if (num_streams > batch) num_streams = batch;
-        const int per = (batch + num_streams - 1) / num_streams;  // the reference drops batch % num_streams
+        C2C &c = c2c_;
+        for (hipStream_t s : c.st) (void)hipStreamSynchronize(s);
+        if (c.plan) mfft_plan_destroy(c.plan);
+        for (auto *v : {&c.dev64, &c.dev32, &c.ws})
+            for (void *p : *v) (void)hipFree(p);
+        for (auto *v : {&c.pin_in, &c.pin_out})
+            for (void *p : *v) (void)hipHostFree(p);
+        for (hipStream_t s : c.st) (void)hipStreamDestroy(s);
+        c2c_ = C2C();
+    }
+
+    void c2c_setup(int x, int y, int per, int nst, int dir)
+    {
+        C2C &c = c2c_;
+        if (c.plan && c.x == x && c.y == y && c.per == per && c.nst == nst && c.dir == dir) return;
+        c2c_release();
         mfft_desc d;
         mfft_desc_default(&d);
         d.precision = MFFT_PREC_SPLIT_FP16;
         d.scale = MFFT_SCALE_NONE;
         d.direction = dir;
         d.in_fmt = d.out_fmt = MFFT_FMT_C32;
-        mfft_plan plan = nullptr;
-        check(mfft_plan_create_2d(&plan, y, x, per, &d), "fft_2d_c2c plan");
+        d.external_workspace = 1;
+        check(mfft_plan_create_2d(&c.plan, y, x, per, &d), "fft_2d_c2c plan");
+        mfft_info inf;
+        check(mfft_plan_info(c.plan, &inf), "fft_2d_c2c plan info");
+        c.x = x;
+        c.y = y;
+        c.per = per;
+        c.nst = nst;
+        c.dir = dir;
+        const size_t block = (size_t)x * y * per;
+        c.st.resize(nst);
+        c.dev64.assign(nst, nullptr);
+        c.dev32.assign(nst, nullptr);
+        c.ws.assign(nst, nullptr);
+        c.pin_in.assign(nst, nullptr);
+        c.pin_out.assign(nst, nullptr);
+        for (int i = 0; i < nst; ++i) {
+            check_hip(hipStreamCreateWithFlags(&c.st[i], hipStreamNonBlocking));
+            check_hip(hipMalloc(&c.dev64[i], block * 16));
+            check_hip(hipMalloc(&c.dev32[i], block * 8));
+            check_hip(hipMalloc(&c.ws[i], inf.workspace_bytes > 0 ? (size_t)inf.workspace_bytes : 16));
+            check_hip(hipHostMalloc(&c.pin_in[i], block * 16, hipHostMallocDefault));
+            check_hip(hipHostMalloc(&c.pin_out[i], block * 16, hipHostMallocDefault));
+        }
+    }
+
+    // Batch split into num_streams blocks, one per stream. Block i: host copy of the
+    // caller's input into pinned memory, then on stream i the copy to the device, fp64 ->
+    // fp32 on the device, the transform (a share of the one plan, the stream's own
+    // workspace), fp32 -> fp64, the copy back to pinned memory; the host copies block i's
+    // result out while later blocks run. Unlike the reference (which drops batch %
+    // num_streams transforms, tensorcore_fft_conv2d.cpp:84) every transform is done.
+    void c2c(const double *input, double *out, int x, int y, int batch, int num_streams, int dir)
+    {
+        if (batch < 1 || x < 2 || y < 2) throw std::runtime_error("fft_2d_c2c: bad sizes");
+        if (num_streams < 1) num_streams = 1;
+        if (num_streams > batch) num_streams = batch;
+        const int per = (batch + num_streams - 1) / num_streams;
+        c2c_setup(x, y, per, num_streams, dir);
+        C2C &c = c2c_;
         const size_t elems = (size_t)x * y;
-        std::vector<hipStream_t> st(num_streams);
-        void *d64 = nullptr, *d32 = nullptr;
-        check_hip(hipMalloc(&d64, elems * batch * 16));
-        check_hip(hipMalloc(&d32, elems * (size_t)per * num_streams * 8));
-        for (auto &s : st) check_hip(hipStreamCreate(&s));
-        // one plan per stream: plans own their workspace
-        std::vector<mfft_plan> sp(num_streams, nullptr);
-        sp[0] = plan;
-        for (int i = 1; i < num_streams; ++i) check(mfft_plan_create_2d(&sp[i], y, x, per, &d), "fft_2d_c2c plan");
-        for (int i = 0; i < num_streams; ++i) {
-            int b0 = i * per, nb = batch - b0 < per ? batch - b0 : per;
-            if (nb <= 0) break;
-            char *h_in = (char *)input + (size_t)b0 * elems * 16;
-            char *g64 = (char *)d64 + (size_t)b0 * elems * 16;
-            char *g32 = (char *)d32 + (size_t)i * per * elems * 8;
-            check_hip(hipMemcpyAsync(g64, h_in, (size_t)nb * elems * 16, hipMemcpyHostToDevice, st[i]));
-            check(mfft_stage_convert(g64, MFFT_FMT_C64, g32, MFFT_FMT_C32, elems, nb, nullptr, 0, nullptr, 0, 1.0, st[i]),
-                  "stage");
-            if (nb == per) {
-                check(mfft_exec(sp[i], g32, g32, st[i]), "exec");
-            } else {
-                mfft_plan tail = nullptr;  // short last block
-                check(mfft_plan_create_2d(&tail, y, x, nb, &d), "tail plan");
-                check(mfft_exec(tail, g32, g32, st[i]), "exec");
-                check_hip(hipStreamSynchronize(st[i]));
-                mfft_plan_destroy(tail);
-            }
-            check(mfft_stage_convert(g32, MFFT_FMT_C32, g64, MFFT_FMT_C64, elems, nb, nullptr, 0, nullptr, 0, 1.0, st[i]),
-                  "stage");
-            check_hip(hipMemcpyAsync((char *)out + (size_t)b0 * elems * 16, g64, (size_t)nb * elems * 16,
-                                     hipMemcpyDeviceToHost, st[i]));
+        const int blocks = (batch + per - 1) / per;
+        for (int i = 0; i < blocks; ++i) {
+            const int b0 = i * per, nb = batch - b0 < per ? batch - b0 : per;
+            const size_t bytes = (size_t)nb * elems * 16;
+            memcpy(c.pin_in[i], (const char *)input + (size_t)b0 * elems * 16, bytes);
+            check_hip(hipMemcpyAsync(c.dev64[i], c.pin_in[i], bytes, hipMemcpyHostToDevice, c.st[i]));
+            check(mfft_stage_convert(c.dev64[i], MFFT_FMT_C64, c.dev32[i], MFFT_FMT_C32, elems, nb, nullptr, 0, nullptr,
+                                     0, 1.0, c.st[i]),
+                  "fft_2d_c2c stage in");
+            check(mfft_exec_ex(c.plan, c.dev32[i], c.dev32[i], nb, c.ws[i], c.st[i]), "fft_2d_c2c exec");
+            check(mfft_stage_convert(c.dev32[i], MFFT_FMT_C32, c.dev64[i], MFFT_FMT_C64, elems, nb, nullptr, 0, nullptr,
+                                     0, 1.0, c.st[i]),
+                  "fft_2d_c2c stage out");
+            check_hip(hipMemcpyAsync(c.pin_out[i], c.dev64[i], bytes, hipMemcpyDeviceToHost, c.st[i]));
         }
-        for (auto &s : st) {
-            check_hip(hipStreamSynchronize(s));
-            check_hip(hipStreamDestroy(s));
+        for (int i = 0; i < blocks; ++i) {
+            const int b0 = i * per, nb = batch - b0 < per ? batch - b0 : per;
+            check_hip(hipStreamSynchronize(c.st[i]));
+            memcpy((char *)out + (size_t)b0 * elems * 16, c.pin_out[i], (size_t)nb * elems * 16);
         }
-        for (mfft_plan p : sp) mfft_plan_destroy(p);
-        check_hip(hipFree(d64));
-        check_hip(hipFree(d32));
     }
 };
 
