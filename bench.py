@@ -387,10 +387,11 @@ def copy_bandwidth(mib=1024, reps=10):
 
 
 def profiled_traffic():
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy
-    kernel of known size), from the summaries committed under profiles/: {key: bytes}, key =
-    "<kernel family>:R<points per line>:c<in bits>>c<out bits>" (conv kernels: "conv:" + the
-    launch-trace name)."""
+    """HBM traffic measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of
+    known size), from the summaries committed under profiles/: {key: (bytes per launch, file)}
+    for key = "<kernel family>:R<points per line>:c<in bits>>c<out bits>", and
+    {"conv:" + launch-trace name: (traffic / algorithmic bytes, file)} for the conv launches,
+    whose sizes depend on the batch chunk."""
     out = {}
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
@@ -403,6 +404,9 @@ def profiled_traffic():
                 if line.startswith("traffic_bytes_per_launch:"):
                     for k, v in json.loads(line.split(":", 1)[1]).items():
                         out[k] = (int(v), name)
+                elif line.startswith("traffic_ratio_per_launch:"):
+                    for k, v in json.loads(line.split(":", 1)[1]).items():
+                        out[k] = (float(v), name)
     return out
 
 
@@ -717,10 +721,12 @@ def conv_config(steps, warmup, with_cpu=True):
         avg_ms, bytes_per = tsum / cnt, bsum / cnt
         achieved = bytes_per / (avg_ms * 1e-3) / 1e9
         all_bytes = sum(rd + wr for _, _, rd, wr in tr)
-        tmeas = profiled_traffic().get("conv:" + nm)
+        tmeas = profiled_traffic().get("conv:" + nm)  # traffic / algorithmic bytes of such a launch
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tmeas[0] if tmeas else None,
-                "traffic_source": f"profiles/{tmeas[1]}" if tmeas else None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": int(tmeas[0] * bytes_per) if tmeas else None,
+                "traffic_source": f"profiles/{tmeas[1]} (measured traffic / algorithmic bytes {tmeas[0]} x this "
+                                  "launch's algorithmic bytes)" if tmeas else None,
                 "kernel": f"{nm} ({cnt} launches per forward)", "algorithmic_bytes_per_launch": int(bytes_per),
                 "avg_launch_ms": round(avg_ms, 5),
                 "launch_time": "HIP events before each launch and after the last (library launch trace), "
