@@ -12,8 +12,8 @@
 //   KIND_ROWS     points contiguous (in_ps = out_ps = 1); lanes walk the points of a line.
 //   KIND_ROWS_TR  points contiguous on input, lines contiguous on output (out_s0 = 1):
 //                 the last stage switches the lanes to walk the lines.
-//   KIND_COLS     lines contiguous on both sides (in_s0 = out_s0 = 1, T = 32 up to 256
-//                 points, 16 at 512, 8 at 1024); lanes walk the lines of the tile
+//   KIND_COLS     lines contiguous on both sides (in_s0 = out_s0 = 1, T = 8192 / R lines);
+//                 lanes walk the lines of the tile
 //                 (strided passes: the first pass of the four-step method, the columns
 //                 of a 2D transform).
 #pragma once
@@ -22,12 +22,15 @@
 namespace mfft {
 
 enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2 };
-// Lines per tile of the strided kind: 32 (a full 128-byte segment of fp16 pairs per point)
-// up to 256 points, then R * T = 8192 points: a 16-wave workgroup keeps two 16-column MFMA
-// groups per wave, whose accumulators, stage values and next-tile operands fit the 128
-// registers a wave has at 4 waves per SIMD (four groups per wave spill to scratch).
+// Lines per tile of the strided kind: R * T = 8192 points at every length (128 lines at 64
+// points, 64 at 128, 32 at 256, 16 at 512, 8 at 1024). A 16-wave workgroup then keeps two
+// 16-column MFMA groups per wave, whose accumulators, stage values and next-tile operands fit
+// the 128 registers a wave has at 4 waves per SIMD (four groups per wave spill to scratch).
+// Each point of a tile is one segment of T consecutive lines in memory, and a strided copy on
+// MI355X streams 1.3-1.6 TB/s with 32-byte segments, 3.0-3.1 with 64, 4.5-4.9 with 128 and
+// 5.0-5.2 with 256 bytes (5.7 for whole rows): short lines get more of them per tile.
 constexpr int kColsMaxR = 1024;
-constexpr int cols_lines(int R) { return R <= 256 ? 32 : 8192 / R; }
+constexpr int cols_lines(int R) { return 8192 / R; }
 
 // Precisions and input formats whose fast kernels also read digit-reversed lines
 // (TileArgs::rev_input): the matrix-core paths with fp16 or fp32 input.

@@ -237,7 +237,7 @@ static int fmt_bytes(int fmt) { return fmt == FMT_C32 ? 8 : 4; }
 
 // ------------------------------------------------------------------ axis planning
 // fast_cols: the pass may run on the strided matrix-core kernel (natural-order input), which
-// holds 16 or 32 lines of up to kColsMaxR points per tile; the tile kernels hold at most
+// holds 8192 / R lines of up to kColsMaxR points per tile; the tile kernels hold at most
 // kMaxTile points, so a strided single pass there needs R <= kMaxTile / 16 for 16 lines.
 static bool single_pass_ok(int R, long long ps_in, const std::vector<Level> &lines, bool fast_cols)
 {
@@ -466,7 +466,7 @@ static int finish_passes(mfft_plan_s *p)
         // reverses digit by digit (every reference list does)
         ps.rev_fast = rev && fast_rev_ok(p->d.precision, in_fmt) && fast_rev_spec(ps.R, ps.rev_list, ps.rev_q);
         // strided lines longer than a 16-line tile kernel holds: the strided matrix-core
-        // kernel when it exists for the formats (16 or 32 lines of up to kColsMaxR points)
+        // kernel when it exists for the formats (8192 / R lines of up to kColsMaxR points)
         const bool cols_long = strided && unit_in >= 0 && ps.R * 16 > kMaxTile && pow2 && ps.R <= kColsMaxR &&
                                (!rev || ps.rev_fast) && ps.lines[unit_in].out_stride == 1 &&
                                find_fast_kernel(p->d.precision, ps.R, KIND_COLS, in_fmt, out_fmt) != nullptr;
