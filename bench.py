@@ -571,17 +571,18 @@ def mfma_utilisation(passes, pass_ms):
     """Matrix-core use of the split-fp16 passes. Every radix-16 stage is a complex DFT16
     on 16 lines at a time: two v_mfma_f32_16x16x32_f16 per real product (the complex
     product is folded into K = 32, the accumulator holds 128 of the 256 complex outputs),
-    three products (Fl*Xh + Fh*Xl + Fh*Xh), two for a first stage that reads exact fp16
-    input (no Xl). The count per launch agrees with rocprofv3's SQ_INSTS_MFMA
-    (profiles/fft_fast_split_n65536x256_mfma.md)."""
+    three products (Fl*Xh + Fh*Xl + Fh*Xh), two for a stage that reads exact fp16 words
+    (no Xl): the first stage of a pass with c16 input, and on the on-chip kernel also the
+    first stage of the second 256-point half (it reads the fp16 intermediate). The count per
+    launch agrees with rocprofv3's SQ_INSTS_MFMA (profiles/fft_fast_split_n65536x256_mfma.md)."""
     out = []
     for p, ms in zip(passes, pass_ms):
-        if p["generic"] or not p["kernel"].startswith("fast"):
+        if p["generic"] or not (p["kernel"].startswith("fast") or p["kernel"] == "onchip"):
             continue
         n16 = sum(1 for r in p["radix"] if r == 16)
         groups = p["points"] // 256
-        exact_first = p["in_elem_bytes"] == 4  # c16 in (the split path's fp16 buffers)
-        per_group = 2 * (2 if exact_first else 3) + 2 * 3 * (n16 - 1) if n16 else 0
+        exact = (2 if p["kernel"] == "onchip" else 1) if p["in_elem_bytes"] == 4 else 0
+        per_group = 2 * (2 * exact + 3 * (n16 - exact)) if n16 else 0
         inst = groups * per_group
         tf = inst * MFMA_FLOPS / (ms * 1e-3) / 1e12
         out.append({"kernel": p["kernel"], "R": p["R"], "instructions_per_launch": inst,

@@ -136,7 +136,9 @@ __device__ __forceinline__ uint32_t pack_cbf16(float re, float im)
 // Split fp32 (re, im) into fp16 pairs hi = fp16(x), lo = fp16(x - hi); x - hi is exact in
 // fp32. The difference is formed as fma(hi, m, x) with m = -1 hidden from the optimiser by
 // an empty asm statement, so it compiles to v_fma_mix_f32 reading the fp16 half in place
-// instead of a conversion back to fp32 and a subtraction.
+// instead of a conversion back to fp32 and a subtraction. (Instructions written in inline
+// asm would get no hazard wait states from the compiler: one that reads an MFMA result right
+// after the MFMA reads a stale accumulator.)
 __device__ __forceinline__ void split_c16(float re, float im, pk_h2 &hi, pk_h2 &lo)
 {
     hi = __builtin_convertvector((pk_f2){re, im}, pk_h2);

@@ -1,0 +1,292 @@
+// On-chip kernel for 65536-point lines: one workgroup transforms a whole line, keeping the
+// 256 x 256 intermediate of the four-step method on the chip, so each line is read from and
+// written to HBM once (the two-pass plan also writes and reads the intermediate: twice the
+// traffic).
+//
+// The method is the reference's four-step decomposition of a long line
+// (tcfft_half.cu:266-390, layer_256_1 followed by layer_256_0), on the MFMA DFT16 of the
+// fast kernels:
+//   x[256 n1 + n2]: phase 1, for each column n2: DFT256 over n1 (two radix-16 stages),
+//                   times W_65536^(n2 k1) -> Y[k1][n2];
+//                   phase 2, for each row k1: DFT256 over n2 -> X[k1 + 256 k2].
+// Storage: Y is 65536 16-bit complex words (256 KB) but a compute unit has 160 KB of LDS,
+// so phase 2 runs in four rounds of 64 rows. Phase 1 writes the rows of round 0 to LDS and
+// keeps the other three rounds in registers (96 words per lane of 512 lanes); round r
+// (r >= 1) first dumps its words into the LDS rows. The DFT16 operand rows are permuted so a
+// lane's four outputs of a group fall one in each round (output row 4g + i holds frequency
+// 4i + g), which spreads the held words evenly over the lanes.
+//
+// Geometry: 8 waves (2 per SIMD, 256 registers each). A stage transforms 32 lines (columns
+// in phase 1, rows in phase 2) = 512 DFT16s = 32 MFMA groups, 4 per wave. Between the two
+// radix-16 stages of a DFT256 the values go through an fp32 exchange buffer (64 KB). Phase-1
+// loads run one 32-column chunk ahead of the compute (16 words per lane). Global loads and
+// stores go through a buffer resource per line (scalar base, 32-bit lane offsets).
+#include "mfft_fast_impl.h"
+#include "mfft_onchip.h"
+
+namespace mfft {
+namespace onchip {
+using fast::f4;
+using fast::h8;
+using fast::b8;
+using fast::Ops;
+using fast::dft16;
+using fast::cmulf;
+using fast::Elem;
+
+constexpr int NW = 8;
+constexpr int NT = NW * 64;
+constexpr int SIDE = 256;        // x[256 n1 + n2]
+constexpr int CH = 32;           // lines per stage (phase-1 columns / phase-2 rows)
+constexpr int NCH = SIDE / CH;   // 8 column chunks
+constexpr int GPW = 4;           // MFMA groups per wave per stage
+constexpr int ROWS = 64;         // rows of Y in LDS at a time (one round)
+constexpr int ROUNDS = SIDE / ROWS;
+// Phase-1 chunks in flight ahead of the compute. One (32 KB per compute unit) measured
+// 1.3 us faster on 65536 x 256 than two or three: the loads are not what limits the kernel
+// (without them it runs only 3 us faster), and the deeper buffers cost registers.
+#ifndef MFFT_ONCHIP_DEPTH
+#define MFFT_ONCHIP_DEPTH 1
+#endif
+constexpr int DEPTH = MFFT_ONCHIP_DEPTH;
+
+// Y rows in LDS: word (row, n2) at row * YP + n2 + 16 (n2 >> 6). With YP = 1 (mod 64) the
+// lanes of a write (16 consecutive n2 x 4 rows 16 apart) and of a read (16 consecutive rows
+// x 4 points 64 apart) fall on 64 distinct banks, and every address is a per-lane base plus
+// a compile-time offset (no per-access index arithmetic, so no address registers held).
+constexpr int YP = SIDE + 3 * 16 + 17;  // 321 words per row
+static_assert(YP % 64 == 1 && YP >= SIDE + 3 * 16, "row pitch");
+
+// Exchange buffer between the radix-16 stages: (a, f, t) = (input digit of the second
+// stage, frequency of the first, line), float2 at [16 a + f][t ^ 16 b] with b = bit 0 of
+// (row ^ row >> 6). Writes (16 consecutive t x 4 consecutive f) and reads (16 consecutive t
+// x 4 values of a, 4 apart) then hit 32 distinct float2 slots per half wave. For the fixed
+// lane and group patterns the swizzle bit is constant per lane (writes: (g ^ u) & 1, reads:
+// (g ^ (wave >> 1)) & 1), so both are per-lane bases plus offsets.
+__host__ __device__ constexpr int ephys(int a, int f, int t)
+{
+    return (16 * a + f) * CH + (t ^ ((((16 * a + f) ^ ((16 * a + f) >> 6)) & 1) << 4));
+}
+
+// W_65536^e from the staged tables: lo (scaled by the stage factor) at [0, 256), hi at
+// [256, 512).
+__device__ __forceinline__ float2 tw65536(const float2 *twl, unsigned e)
+{
+    e &= 65535u;
+    return cmulf(twl[e & 255u], twl[256 + (e >> 8)]);
+}
+
+template <int PREC, int FMT>
+struct Chunk {
+    typedef typename Elem<FMT>::raw raw;
+    raw w[GPW * 4];
+};
+
+// Stage 1 of a DFT256 over 32 lines: group u of this wave transforms digit a = (wave >> 1) +
+// 4u of lines t = 16 (wave & 1) + c; the lane's four inputs are points a + 16 (4g + i) in the
+// 16-bit words `w` (exact MFMA operands). Results times W_256^(a f) (scaled) go to the
+// exchange buffer at (a, f, t), f = 4i + g. The caller has passed a barrier since the
+// previous stage 2 read the buffer; each group's products are written as they complete, so
+// only one group's accumulators are live.
+template <int PREC, int FMT>
+__device__ __forceinline__ void stage1(const typename Elem<FMT>::raw *w, const Ops<PREC> &op, const float2 *stw,
+                                       float2 *ex, int w2, int g, const int *exw)
+{
+#pragma unroll
+    for (int u = 0; u < GPW; ++u) {
+        fast::u4 v = {w[4 * u], w[4 * u + 1], w[4 * u + 2], w[4 * u + 3]};
+        const float2 x[4] = {};
+        f4 ar, ai;
+        dft16<PREC, true>(x, __builtin_bit_cast(h8, v), __builtin_bit_cast(b8, v), op, ar, ai);
+        const int a = w2 + 4 * u;
+        const float2 *sw = stw + a * g;  // W_256^(a f), f = 4i + g: index a g + 4 a i
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            ex[exw[u & 1] + (64 * u + 4 * i) * CH] = cmulf(make_float2(ar[i], ai[i]), sw[4 * a * i]);
+    }
+    __syncthreads();
+}
+
+// Stage 2, group u: transforms frequency f = (wave >> 1) + 4u of lines t = 16 (wave & 1) + c
+// over a; the lane reads a = 4g + i and gets outputs v = 4i + g (point f + 16 v of the line).
+// exr: the lane's read base (a = 4g, f = wave >> 1).
+template <int PREC>
+__device__ __forceinline__ void stage2(const float2 *ex, const Ops<PREC> &op, int u, int exr, f4 &ar, f4 &ai)
+{
+    float2 x[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = ex[exr + (16 * i + 4 * u) * CH];
+    dft16<PREC, false>(x, h8{}, b8{}, op, ar, ai);
+}
+
+// Buffer resource of one line (gfx9 descriptor word 3; the record count is the line's bytes,
+// so the hardware drops any access outside the line). Loads and stores through it take the
+// lane's 32-bit byte offset and a uniform byte offset in a scalar register: no 64-bit address
+// arithmetic per access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t line_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, kOnchipN * 4, 0x00020000);
+}
+
+// Chunk j of phase 1: point n1 = a + 16 (4g + i) of column n2 = 32 j + t. 16 lanes read 16
+// consecutive columns (64 bytes), the two waves of a pair the two halves of a 128-byte row
+// segment; `wave` is uniform, so the row part of the offset is scalar.
+template <int PREC, int FMT>
+__device__ __forceinline__ void load_chunk(Chunk<PREC, FMT> &ck, __amdgpu_buffer_rsrc_t src, int j, int wave,
+                                           unsigned lane_off)
+{
+#pragma unroll
+    for (int u = 0; u < GPW; ++u) {
+        const int a = (wave >> 1) + 4 * u;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            ck.w[4 * u + i] =
+                __builtin_amdgcn_raw_buffer_load_b32(src, (int)(4 * lane_off), 4 * (SIDE * (a + 16 * i) + CH * j), 0);
+    }
+}
+
+template <int PREC, int FMT>
+__global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
+{
+    typedef typename Elem<FMT>::raw raw;
+    static_assert(sizeof(raw) == 4, "16-bit complex words");
+    __shared__ raw ylds[ROWS * YP];        // 80 KB: the rows of one round
+    __shared__ float2 ex[SIDE * CH];       // 64 KB: exchange between radix-16 stages
+    __shared__ float2 stw[SIDE];           // W_256^e, scaled
+    __shared__ float2 twl[2 * SIDE];       // W_65536 lo (scaled) and hi halves
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w0 = wave & 1, w2 = wave >> 1;
+    const int g = lane >> 4, c = lane & 15;
+    const int t = 16 * w0 + c;                                // the lane's line in a stage
+    const unsigned lane_off = (unsigned)(SIDE * 64 * g + t);  // phase-1 load offset, lane part
+    // exchange-buffer bases: stage-1 writes for even / odd u, stage-2 reads
+    // (the odd base is taken at u = 1 and moved back by u = 1's offset, 64 rows)
+    const int exw[2] = {ephys(w2, g, t), ephys(w2 + 4, g, t) - 64 * CH};
+    const int exr = ephys(4 * g, w2, t);
+    // Y bases: writes (row f + 16 g, n2 = 32 j + t), reads (row 32 hb + t, n2 = a2 + 16 i + 64 g)
+    const int yw = 16 * g * YP + c;
+    const int yr = c * YP + 80 * g;
+    const float fac = a.fac, inv = 1.0f / a.fac;
+
+    Ops<PREC> op;
+    // permuted operand rows: output row m of a group holds frequency 4 (m % 4) + m / 4
+    const int m = lane & 15;
+    op.load(reinterpret_cast<const DftOps *>(a.dft_ops), (lane & 48) | (4 * (m & 3) + (m >> 2)));
+
+    // one line per workgroup (grid = lines): a loop over lines would make every lane address
+    // loop-invariant, and hoisting them all out of it exhausts the registers
+    const long long b = blockIdx.x;
+    const __amdgpu_buffer_rsrc_t src = line_rsrc(reinterpret_cast<const raw *>(a.in) + b * a.in_dist);
+    const __amdgpu_buffer_rsrc_t dst = line_rsrc(reinterpret_cast<raw *>(a.out) + b * a.out_dist);
+    // the tables are read before the first chunks are requested and written to LDS after,
+    // so their wait does not cover the chunks' loads
+    const bool tab = threadIdx.x < SIDE;
+    float2 t0, t1, t2;
+    if (tab) {
+        t0 = a.stage_tw_s[threadIdx.x];
+        t1 = a.tw_lo_s[threadIdx.x];
+        t2 = a.tw_hi[threadIdx.x];
+    }
+    Chunk<PREC, FMT> buf[DEPTH + 1];
+#pragma unroll
+    for (int j = 0; j < DEPTH; ++j) load_chunk<PREC, FMT>(buf[j], src, j, wave, lane_off);
+    if (tab) {
+        stw[threadIdx.x] = t0;
+        twl[threadIdx.x] = t1;
+        twl[SIDE + threadIdx.x] = t2;
+    }
+    // rounds 1..3 of Y, held: hold[j][u][r - 1] is point (k1 = f + 16 g + 64 r, n2 = 32 j + t)
+    raw hold[NCH][GPW][ROUNDS - 1];
+
+    // ---------------- phase 1: columns
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+        if (j + DEPTH < NCH) {
+            // issued before this chunk's stage 1 reads its buffer: buf[(j + DEPTH) % (DEPTH
+            // + 1)] was consumed by chunk j - 1
+            load_chunk<PREC, FMT>(buf[(j + DEPTH) % (DEPTH + 1)], src, j + DEPTH, wave, lane_off);
+        }
+        __syncthreads();  // the previous chunk's stage 2 has read the exchange buffer
+        stage1<PREC, FMT>(buf[j % (DEPTH + 1)].w, op, stw, ex, w2, g, exw);
+        // four-step twiddle W^(n2 k1), k1 = f + 16 g + 64 i: one lookup for i = 0 and
+        // the step W^(64 n2) for the rest
+        const unsigned n2 = (unsigned)(CH * j + t);
+        float2 st = tw65536(twl, 64u * n2);
+        st.x *= inv;
+        st.y *= inv;
+#pragma unroll
+        for (int u = 0; u < GPW; ++u) {
+            const int f = w2 + 4 * u;
+            f4 ar, ai;
+            stage2<PREC>(ex, op, u, exr, ar, ai);
+            float2 w = tw65536(twl, n2 * (unsigned)(f + 16 * g));
+            // Y word (row f + 16 g, n2): base + f YP + 32 j + 16 w0 + 16 (j >> 1)
+            const int yo = f * YP + CH * j + 16 * w0 + 16 * (j >> 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 y = cmulf(make_float2(ar[i], ai[i]), w);
+                if (i == 0)
+                    ylds[yw + yo] = Elem<FMT>::pack(y);
+                else
+                    hold[j][u][i - 1] = Elem<FMT>::pack(y);
+                if (i < 3) w = cmulf(w, st);
+            }
+        }
+    }
+
+    // ---------------- phase 2: rows, four rounds of 64
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        if (r > 0) {
+            // the previous round's stage-1 reads of these rows ended at the barrier
+            // before its last exchange write
+#pragma unroll
+            for (int j = 0; j < NCH; ++j)
+#pragma unroll
+                for (int u = 0; u < GPW; ++u)
+                    ylds[yw + (w2 + 4 * u) * YP + CH * j + 16 * w0 + 16 * (j >> 1)] = hold[j][u][r - 1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int hb = 0; hb < 2; ++hb) {
+            // rows 32 hb + t of the round; stage-1 operands: points a2 + 16 (4g + i)
+            raw w[GPW * 4];
+#pragma unroll
+            for (int u = 0; u < GPW; ++u)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    w[4 * u + i] = ylds[yr + (CH * hb + 16 * w0) * YP + w2 + 4 * u + 16 * i];
+            __syncthreads();  // the previous block's stage 2 has read the exchange buffer
+            stage1<PREC, FMT>(w, op, stw, ex, w2, g, exw);
+            // X[k1 + 256 k2], k1 = 64 r + 32 hb + t, k2 = f + 16 (4i + g): 16 lanes store 16
+            // consecutive k1; uniform part 64 r + 32 hb + 256 (f + 64 i), lane part t + 4096 g
+            const unsigned st_off = (unsigned)(t + SIDE * 16 * g);
+#pragma unroll
+            for (int u = 0; u < GPW; ++u) {
+                const int f = w2 + 4 * u;
+                f4 ar, ai;
+                stage2<PREC>(ex, op, u, exr, ar, ai);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(make_float2(ar[i] * fac, ai[i] * fac)), dst,
+                                                          (int)(4 * st_off),
+                                                          4 * (ROWS * r + CH * hb + SIDE * (f + 64 * i)), 0);
+            }
+        }
+    }
+}
+
+}  // namespace onchip
+
+const void *find_onchip_kernel(int prec, int fmt)
+{
+    using namespace onchip;
+    if (prec == PREC_SPLIT && fmt == FMT_C16) return (const void *)fft_onchip<PREC_SPLIT, FMT_C16>;
+    if (prec == PREC_FP16 && fmt == FMT_C16) return (const void *)fft_onchip<PREC_FP16, FMT_C16>;
+    if (prec == PREC_BF16 && fmt == FMT_CBF16) return (const void *)fft_onchip<PREC_BF16, FMT_CBF16>;
+    return nullptr;
+}
+
+}  // namespace mfft

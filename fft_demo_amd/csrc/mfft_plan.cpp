@@ -12,6 +12,7 @@
 #include "mfft_common.h"
 #include "mfft_fast.h"
 #include "mfft_internal.h"
+#include "mfft_onchip.h"
 
 #include <hip/hip_ext.h>
 
@@ -211,6 +212,7 @@ struct PassSpec {
     int *perm = nullptr;
     bool rev_fast = false;  // the fast kernel reads the digit-reversed lines (rev_q)
     DigitRev rev_q{};
+    bool onchip = false;    // the whole 65536-point line in one pass (fft_onchip)
 };
 
 struct mfft_plan_s {
@@ -380,6 +382,33 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, a_dst, dst, -1, 1, narrow);
 }
 
+// 65536-point lines in natural order with 16-bit words in, out and between the two 256-point
+// halves: one pass of the on-chip kernel (the line is read and written once) instead of the
+// four-step pair, unless a split is asked for (split_hint > 1). The halves are the same
+// four-step decomposition (tcfft_half.cu:266-390 then 69-161 for the reference's long lines).
+static bool plan_onchip(mfft_plan_s *p, const mfft_desc *d)
+{
+    if (p->n != kOnchipN || d->in_order != MFFT_ORDER_NATURAL || d->split_hint > 1) return false;
+    const int fmt = d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16 : MFFT_FMT_C16;
+    const int mid = d->mid_fmt >= 0                         ? d->mid_fmt
+                    : d->precision == MFFT_PREC_FP16 ? MFFT_FMT_C16
+                    : d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16
+                                                     : MFFT_FMT_C32;
+    if (d->in_fmt != fmt || d->out_fmt != fmt || mid != fmt) return false;
+    if (!find_onchip_kernel(d->precision, fmt)) return false;
+    PassSpec ps;
+    ps.R = kOnchipN;
+    ps.ps_in = ps.ps_out = 1;
+    ps.lines = {Level{p->batch, kOnchipN, kOnchipN, true}};
+    ps.tw_N = kOnchipN;  // the four-step tables between the halves
+    ps.tw_level = 0;
+    ps.src = BUF_IN;
+    ps.dst = BUF_OUT;
+    ps.onchip = true;
+    p->passes.push_back(ps);
+    return true;
+}
+
 // Digit-reversed input (tcfftExec's order, tcfft_half.cu:1177-1233) of a 1D length longer
 // than one tile: N = L1 L2, split after the first m radices of the list. Position
 // p = p_lo + L1 p_hi holds natural element n2 + L2 n1 with n1 = gen_rev(list[:m])[p_lo] and
@@ -448,6 +477,16 @@ static int plan_reversed_1d(mfft_plan_s *p)
 static int finish_passes(mfft_plan_s *p)
 {
     for (auto &ps : p->passes) {
+        if (ps.onchip) {
+            ps.radix = {16, 16, 16, 16};
+            ps.fast_kind = KIND_ONCHIP;
+            ps.fn = find_onchip_kernel(p->d.precision, p->d.in_fmt);
+            ps.l0 = 0;
+            ps.T = 1;
+            ps.TL = 1;
+            if (!ps.fn) return fail(MFFT_ERR_UNSUPPORTED, "no on-chip kernel for the formats");
+            continue;
+        }
         if (!line_radices(ps.R, ps.radix)) return fail(MFFT_ERR_UNSUPPORTED, "line length %d", ps.R);
         if (ps.perm_axis >= 0 && ps.rev_list.empty()) ps.rev_list = p->rev_list_axis[ps.perm_axis];
         const bool rev = !ps.rev_list.empty();
@@ -589,8 +628,10 @@ static int build_tables(mfft_plan_s *p)
     if (alloc_dev(p, (void **)&p->ops, sizeof(DftOps))) return MFFT_ERR_NOMEM;
     HIP_TRY(gen_dft_ops(p->ops, sign, 0));
     for (auto &ps : p->passes) {
-        if (alloc_dev(p, (void **)&ps.stage_tw, sizeof(float2) * ps.R)) return MFFT_ERR_NOMEM;
-        HIP_TRY(gen_twiddle(ps.stage_tw, ps.R, ps.R, 1, sign, 0));
+        // the on-chip kernel's stages are 256-point halves
+        const int Rst = ps.onchip ? 256 : ps.R;
+        if (alloc_dev(p, (void **)&ps.stage_tw, sizeof(float2) * Rst)) return MFFT_ERR_NOMEM;
+        HIP_TRY(gen_twiddle(ps.stage_tw, Rst, Rst, 1, sign, 0));
         if (ps.tw_N) {
             int bits = 0;
             while ((1LL << (2 * bits)) < ps.tw_N) ++bits;
@@ -610,8 +651,8 @@ static int build_tables(mfft_plan_s *p)
             int tail = 1;
             for (int r : ps.radix) if (r != 16) tail = r;
             ps.tail_mul = (p->d.scale == MFFT_SCALE_STAGE) ? 1.0f / tail : 1.0f;
-            if (alloc_dev(p, (void **)&ps.stage_tw_s, sizeof(float2) * ps.R)) return MFFT_ERR_NOMEM;
-            HIP_TRY(gen_twiddle(ps.stage_tw_s, ps.R, ps.R, 1, sign, 0, fac));
+            if (alloc_dev(p, (void **)&ps.stage_tw_s, sizeof(float2) * Rst)) return MFFT_ERR_NOMEM;
+            HIP_TRY(gen_twiddle(ps.stage_tw_s, Rst, Rst, 1, sign, 0, fac));
             if (ps.tw_N) {
                 long long nlo = 1LL << ps.tw_bits;
                 if (alloc_dev(p, (void **)&ps.tw_lo_s, sizeof(float2) * nlo)) return MFFT_ERR_NOMEM;
@@ -690,6 +731,7 @@ static void trace_pass(const mfft_plan_s *p, const PassSpec &ps, const TileArgs 
                       : ps.fast_kind == KIND_ROWS ? "fast_rows"
                       : ps.fast_kind == KIND_ROWS_TR ? "fast_rows_tr"
                       : ps.fast_kind == KIND_COLS ? "fast_cols"
+                      : ps.fast_kind == KIND_ONCHIP ? "onchip"
                       : ps.generic ? "tile_generic" : "tile";
     const long long pts = (long long)a.c0 * a.c1 * a.c2 * ps.R;
     long long rd = pts * fmt_bytes(a.in_fmt);
@@ -794,6 +836,27 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             if (kstart || kstop) return hipExtLaunchKernel(ps.fn, grid, block, args, 0, s, kstart, kstop, 0);
             return hipLaunchKernel(ps.fn, grid, block, args, 0, s);
         };
+        if (ps.fast_kind == KIND_ONCHIP) {
+            // one workgroup per line
+            OnchipArgs oa;
+            oa.in = a.in;
+            oa.out = a.out;
+            oa.batch = nb;
+            oa.in_dist = ps.lines[0].in_stride;
+            oa.out_dist = ps.lines[0].out_stride;
+            oa.dft_ops = p->ops;
+            oa.stage_tw_s = ps.stage_tw_s;
+            oa.tw_lo_s = ps.tw_lo_s;
+            oa.tw_hi = ps.tw_hi;
+            oa.fac = ps.last_mul;
+            void *oargs[] = {&oa};
+            const dim3 grid((unsigned)nb), block(kOnchipThreads);
+            if (kstart || kstop)
+                HIP_TRY(hipExtLaunchKernel(ps.fn, grid, block, oargs, 0, s, kstart, kstop, 0));
+            else
+                HIP_TRY(hipLaunchKernel(ps.fn, grid, block, oargs, 0, s));
+            continue;
+        }
         if (ps.small) {
             long long nl = (long long)a.c0 * a.c1 * a.c2;
             HIP_TRY(launch(dim3((unsigned)((nl + 255) / 256)), dim3(256)));
@@ -883,6 +946,8 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     }
     if (rev && n > kMaxTile) {
         rc = plan_reversed_1d(p);
+    } else if (plan_onchip(p, d)) {
+        rc = MFFT_OK;
     } else {
         std::vector<Level> lines{Level{batch, n, n, true}};
         rc = plan_axis(p->passes, n, 1, 1, lines, BUF_IN, BUF_OUT, rev ? 0 : -1, d->split_hint, narrow_mid(d));
@@ -1000,6 +1065,7 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
                     : ps.fast_kind == KIND_ROWS ? MFFT_KERNEL_ROWS
                     : ps.fast_kind == KIND_ROWS_TR ? MFFT_KERNEL_ROWS_TR
                     : ps.fast_kind == KIND_COLS ? MFFT_KERNEL_COLS
+                    : ps.fast_kind == KIND_ONCHIP ? MFFT_KERNEL_ONCHIP
                     : ps.generic ? MFFT_KERNEL_GENERIC : MFFT_KERNEL_TILE;
     }
     info->out_scale = p->out_scale;

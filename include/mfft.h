@@ -98,7 +98,8 @@ enum {
     MFFT_KERNEL_ROWS = 2,      /* matrix-core kernel, contiguous lines in and out (fft_fast) */
     MFFT_KERNEL_ROWS_TR = 3,   /* matrix-core kernel, contiguous in, transposed out (fft_fast) */
     MFFT_KERNEL_COLS = 4,      /* matrix-core kernel, strided 256-point lines (fft_fast) */
-    MFFT_KERNEL_STREAM = 5     /* streaming short-DFT kernel, one thread per line (small_cols) */
+    MFFT_KERNEL_STREAM = 5,    /* streaming short-DFT kernel, one thread per line (small_cols) */
+    MFFT_KERNEL_ONCHIP = 6     /* one workgroup per 65536-point line, the intermediate on chip (fft_onchip) */
 };
 
 typedef struct mfft_info {

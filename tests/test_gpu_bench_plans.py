@@ -2,10 +2,11 @@
 against the FP64 oracle (the reference's accuracy check, main.cpp:164-198, on the
 workloads the throughput numbers are quoted for).
 
-  * 1D N = 65536 x 256, split fp16 with fp16 intermediates between the two passes (the
-    headline plan: a strided 256-point pass with the four-step twiddle, then a contiguous
-    256-point pass with a transposed store), under the planner's schedule and under the
-    autotuned one; the inverse; fp16 and bf16 plans of the same size;
+  * 1D N = 65536 x 256, split fp16 with fp16 intermediates (the headline plan: the
+    planner's one-pass on-chip kernel, and the two-pass schedule -- a strided 256-point pass
+    with the four-step twiddle, then a contiguous 256-point pass with a transposed store --
+    asked for with split_hint = 256), and under the autotuned schedule; the inverse; fp16
+    and bf16 plans of the same size;
   * 2D 1024 x 1024 x 64 with 10% of the entries x100, split fp16 with fp16 intermediates:
     forward against the oracle, the inverse of that output against the oracle's inverse,
     and the round trip against the input;
@@ -46,40 +47,47 @@ def _spiky_input(shape, batch, seed=42):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tuned", [False, True])
-def test_c2_split_c16_mid_full_batch(dev, tuned):
-    """The headline plan on the whole 65536 x 256 batch: passes (fast_cols 256, c16 -> c16)
-    then (fast_rows_tr 256, c16 -> c16), every transform against the oracle."""
+@pytest.mark.parametrize("schedule", ["planner", "two_pass", "tuned"])
+def test_c2_split_c16_mid_full_batch(dev, schedule):
+    """The headline transform on the whole 65536 x 256 batch, every transform against the
+    oracle: the planner's plan (one pass of the on-chip kernel, c16 -> c16), the two-pass
+    plan (fast_cols 256 then fast_rows_tr 256, c16 throughout) and the autotuned one."""
     import fft_demo_amd as F
     from fft_demo_amd import tune
 
     n, batch = 65536, 256
     kw = dict(precision="split_fp16", mid_fmt="c16")
-    if tuned:
+    if schedule == "tuned":
         e = tune.autotune((n,), batch, **kw)
         print("tuned schedule", e)
         kw["split_hint"] = tune.SPLIT_TUNED
+    elif schedule == "two_pass":
+        kw["split_hint"] = 256
     plan = F.Plan((n,), batch, **kw)
     kinds = _kinds(plan)
     print("passes", kinds, "chunk", plan.info["batch_chunk"])
-    if not tuned:
+    if schedule == "planner":
+        assert kinds == [("onchip", n, 4, 4)], kinds
+    elif schedule == "two_pass":
         assert kinds == [("fast_cols", 256, 4, 4), ("fast_rows_tr", 256, 4, 4)], kinds
     assert all(k[2] == 4 and k[3] == 4 for k in kinds)  # fp16 buffers throughout
     x = complex_input((batch, n), 42)
     y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
     l2, mx = rel_errors(y, oracle.fft(x))
-    print(f"65536 x 256 split c16 mid tuned={tuned}: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    print(f"65536 x 256 split c16 mid {schedule}: rel_l2={l2:.3e} rel_max={mx:.3e}")
     assert l2 <= 1e-3 and mx <= 4e-3
     plan.close()
 
 
 @pytest.mark.gpu
-def test_c2_split_c16_mid_inverse(dev):
+@pytest.mark.parametrize("split", [0, 256])
+def test_c2_split_c16_mid_inverse(dev, split):
     import fft_demo_amd as F
 
     n, batch = 65536, 16
-    plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt="c16", direction=1)
-    assert [k[:2] for k in _kinds(plan)] == [("fast_cols", 256), ("fast_rows_tr", 256)]
+    plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt="c16", direction=1, split_hint=split)
+    want = [("onchip", n)] if split == 0 else [("fast_cols", 256), ("fast_rows_tr", 256)]
+    assert [k[:2] for k in _kinds(plan)] == want
     x = complex_input((batch, n), 43)
     y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
     l2, mx = rel_errors(y, oracle.fft(x, sign=1))

@@ -123,16 +123,17 @@ def test_short_line_kernels_all_precisions(dev, n, prec):
 @pytest.mark.parametrize("n,batch,mid,timed", [(65536, 8, "c16", True), (1 << 20, 2, None, False)])
 def test_exec_pass_by_pass_equals_exec(dev, n, batch, mid, timed):
     """mfft_exec_pass run for every pass in order (the per-kernel timing entry point) leaves
-    exactly the output of one mfft_exec (bit for bit), on a two-pass and a longer plan; with
-    start/stop events (kernel launched through hipExtLaunchKernel) each pass's window is a
-    positive time."""
+    exactly the output of one mfft_exec (bit for bit), on a two-pass plan (65536 with the
+    split asked for: the planner's default there is the one-pass on-chip kernel) and a longer
+    plan; with start/stop events (kernel launched through hipExtLaunchKernel) each pass's
+    window is a positive time."""
     import torch
 
     import fft_demo_amd as F
     from fft_demo_amd import _lib
     from fft_demo_amd.hiptime import EventPool
 
-    plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt=mid)
+    plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt=mid, split_hint=256 if n == 65536 else 0)
     x = to_device(complex_input((batch, n), 23, "c16"), plan.in_dtype(), dev)
     want = plan.exec(x)
     got = plan.empty_out(device=dev)

@@ -1,0 +1,123 @@
+"""GPU parity of the on-chip kernel for 65536-point lines (fft_onchip: one workgroup per line,
+the 256 x 256 four-step intermediate kept in LDS and registers), which the planner picks for
+natural-order 65536-point lines with 16-bit words in, out and between the halves.
+
+Against the FP64 oracle (the reference's check, main.cpp:164-198): split fp16, fp16 and bf16,
+forward and inverse, per-stage and no scaling, batches of 1, 5 and 300 lines (more lines than
+the device runs at once), in place; and against the two-pass plan of the same transform
+(split_hint = 256: the strided 256-point pass with the four-step twiddle, then the contiguous
+one), which computes the same decomposition in the same arithmetic with the intermediate in
+HBM. Inputs: re/im uniform[-1,1] from SplitMix64, rounded to the input format.
+Tolerances (relative L2 / relative max over all real components): split fp16 1e-3 / 4e-3,
+fp16 4e-3 / 2e-2, bf16 3e-2 / 1e-1.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import complex_input, from_device, rel_errors, to_device
+
+N = 65536
+TOL = {"split_fp16": (1e-3, 4e-3), "fp16": (4e-3, 2e-2), "bf16": (3e-2, 1e-1)}
+FMT = {"split_fp16": "c16", "fp16": "c16", "bf16": "cbf16"}
+
+
+def _plan(prec, batch, **kw):
+    import fft_demo_amd as F
+
+    if prec == "split_fp16":
+        kw.setdefault("mid_fmt", "c16")
+    return F.Plan((N,), batch, precision=prec, **kw)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp16", "bf16"])
+@pytest.mark.parametrize("direction", [-1, 1])
+@pytest.mark.parametrize("batch", [1, 5, 300])
+def test_onchip_vs_oracle(dev, prec, direction, batch):
+    plan = _plan(prec, batch, direction=direction)
+    info = plan.info
+    assert [(p["kernel"], p["R"]) for p in info["passes"]] == [("onchip", N)], info["passes"]
+    assert info["workspace_bytes"] == 0 and info["launches_per_exec"] == 1
+    x = complex_input((batch, N), 1000 + 7 * batch + direction, FMT[prec])
+    y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
+    l2, mx = rel_errors(y, oracle.fft(x, sign=direction))
+    print(f"onchip {prec} dir={direction} b={batch}: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+    plan.close()
+
+
+@pytest.mark.gpu
+def test_onchip_unscaled(dev):
+    """scale = 0 (no per-stage scaling): the output is the plain DFT."""
+    plan = _plan("split_fp16", 3, scale=0)
+    assert plan.info["passes"][0]["kernel"] == "onchip" and plan.out_scale == 1.0
+    x = complex_input((3, N), 31, "c16", -1 / 64, 1 / 64)  # |DFT| <= N / 64 * sqrt(2): inside fp16
+    y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev)))
+    l2, mx = rel_errors(y, oracle.fft(x))
+    print(f"onchip unscaled: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= 1e-3 and mx <= 4e-3
+    plan.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "bf16"])
+def test_onchip_matches_two_pass(dev, prec):
+    """The same transform through the two-pass plan (intermediate rounded to the same
+    16-bit format in HBM): the two outputs agree to well within either's error bound."""
+    import torch
+
+    b = 16
+    one = _plan(prec, b)
+    two = _plan(prec, b, split_hint=256)
+    assert [p["kernel"] for p in two.info["passes"]] == ["fast_cols", "fast_rows_tr"]
+    x = complex_input((b, N), 55, FMT[prec])
+    xd = to_device(x, one.in_dtype(), dev)
+    y1 = from_device(one.exec(xd))
+    y2 = from_device(two.exec(xd))
+    torch.cuda.synchronize()
+    l2, mx = rel_errors(y1, y2)
+    ref = oracle.fft(x) * one.out_scale
+    e1, _ = rel_errors(y1, ref)
+    e2, _ = rel_errors(y2, ref)
+    print(f"{prec}: onchip vs two-pass rel_l2={l2:.3e} rel_max={mx:.3e}; vs oracle {e1:.3e} / {e2:.3e}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+    one.close()
+    two.close()
+
+
+@pytest.mark.gpu
+def test_onchip_in_place(dev):
+    """Input and output in one buffer (each workgroup reads its whole line before it writes
+    any of it): equal bit for bit to the out-of-place result."""
+    import torch
+
+    b = 7
+    plan = _plan("split_fp16", b)
+    x = complex_input((b, N), 61)
+    xd = to_device(x, plan.in_dtype(), dev)
+    want = plan.exec(xd.clone())
+    plan.exec(xd, xd)
+    torch.cuda.synchronize()
+    assert torch.equal(xd, want)
+    plan.close()
+
+
+@pytest.mark.gpu
+def test_onchip_impulses(dev):
+    """Closed forms: a unit impulse at n0 transforms to exp(-2 pi i n0 k / N) (no scaling: every
+    intermediate has magnitude 1); n0 chosen to exercise every digit position of the
+    256 x 256 split."""
+    plan = _plan("split_fp16", 4, scale=0)
+    n0s = [0, 1, 257, 40961]
+    x = np.zeros((4, N), complex)
+    for i, n0 in enumerate(n0s):
+        x[i, n0] = 1.0
+    y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev)))
+    k = np.arange(N)
+    for i, n0 in enumerate(n0s):
+        ref = np.exp(-2j * np.pi * ((n0 * k) % N) / N)
+        err = np.abs(y[i] - ref).max()
+        print(f"impulse at {n0}: max abs error = {err:.3e}")
+        assert err <= 2e-3
+    plan.close()

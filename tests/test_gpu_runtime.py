@@ -107,7 +107,7 @@ def test_two_streams_share_plan_workspace(dev):
     import fft_demo_amd as F
 
     n, b = 65536, 8
-    plan = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16")
+    plan = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16", split_hint=256)  # the two-pass plan
     assert plan.info["workspace_bytes"] > 0
     xs = [to_device(complex_input((b, n), 60 + k), torch.float16, dev) for k in range(4)]
     want = [plan.exec(x).clone() for x in xs]

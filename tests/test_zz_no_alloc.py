@@ -24,9 +24,11 @@ def _noalloc(lib, plan, x, out, batch, stream):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape,batch,small,mid", [((65536,), 32, 5, "c16"), ((1024, 1024), 8, 3, None),
-                                                   ((4096,), 64, 17, None)])
-def test_exec_noalloc_matches_exec(dev, shape, batch, small, mid):
+@pytest.mark.parametrize("shape,batch,small,mid,split", [((65536,), 32, 5, "c16", 256), ((65536,), 32, 5, "c16", 0),
+                                                         ((1024, 1024), 8, 3, None, 0), ((4096,), 64, 17, None, 0)])
+def test_exec_noalloc_matches_exec(dev, shape, batch, small, mid, split):
+    """split = 256 at 65536 points: the two-pass plan (intermediates in the scratch array);
+    split = 0 there: the one-pass on-chip plan, which has no intermediates in memory."""
     import torch
 
     import fft_demo_amd as F
@@ -34,7 +36,7 @@ def test_exec_noalloc_matches_exec(dev, shape, batch, small, mid):
 
     lib = L.load()
     # the plan keeps its intermediates within the scratch array (a 32 MiB chunk)
-    plan = F.Plan(shape, batch, precision="split_fp16", mid_fmt=mid, workspace_limit=32 << 20)
+    plan = F.Plan(shape, batch, precision="split_fp16", mid_fmt=mid, workspace_limit=32 << 20, split_hint=split)
     assert plan.info["workspace_bytes"] <= SCRATCH_BYTES
     s = torch.cuda.current_stream(dev)
     x = complex_input((batch, *shape), 71)
