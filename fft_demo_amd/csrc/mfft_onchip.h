@@ -21,7 +21,8 @@ struct OnchipArgs {
 constexpr int kOnchipN = 65536;
 constexpr int kOnchipThreads = 512;
 
-// The kernel for a precision and its 16-bit format, or null.
-const void *find_onchip_kernel(int prec, int fmt);
+// The kernel for a precision and its 16-bit format, or null. rev: the lines are stored in the
+// digit-reversed order of four radix-16 digits (split fp16 and fp16 only).
+const void *find_onchip_kernel(int prec, int fmt, bool rev);
 
 }  // namespace mfft

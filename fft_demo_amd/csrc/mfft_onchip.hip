@@ -131,21 +131,35 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t line_rsrc(const void *base)
 // Chunk j of phase 1: point n1 = a + 16 (4g + i) of column n2 = 32 j + t. 16 lanes read 16
 // consecutive columns (64 bytes), the two waves of a pair the two halves of a 128-byte row
 // segment; `wave` is uniform, so the row part of the offset is scalar.
-template <int PREC, int FMT>
+//
+// REV: the line is stored in the digit-reversed order of four radix-16 digits (gen_rev of
+// {16, 16, 16, 16}, tcfft_half.cu:1351-1374: the order tcfftExec reads), where natural point
+// 256 n1 + n2 sits at rev(n1) + 256 rev(n2), rev swapping the two hex digits of a byte. The
+// lane's four points n1 = a + 16 (4g + i) then sit at 4g + i + 16 a + 256 rev(n2): one
+// 16-byte load, with rev(32 j + 16 (wave & 1) + c) = 2 j + (wave & 1) + 16 c. A chunk reads
+// 32 whole 1 KB rows of the stored line.
+template <int PREC, int FMT, bool REV>
 __device__ __forceinline__ void load_chunk(Chunk<PREC, FMT> &ck, __amdgpu_buffer_rsrc_t src, int j, int wave,
                                            unsigned lane_off)
 {
 #pragma unroll
     for (int u = 0; u < GPW; ++u) {
         const int a = (wave >> 1) + 4 * u;
+        if constexpr (REV) {
+            const fast::u4 v = __builtin_amdgcn_raw_buffer_load_b128(
+                src, (int)(4 * lane_off), 4 * (16 * a + SIDE * (2 * j + (wave & 1))), 0);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            ck.w[4 * u + i] =
-                __builtin_amdgcn_raw_buffer_load_b32(src, (int)(4 * lane_off), 4 * (SIDE * (a + 16 * i) + CH * j), 0);
+            for (int i = 0; i < 4; ++i) ck.w[4 * u + i] = v[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                ck.w[4 * u + i] = __builtin_amdgcn_raw_buffer_load_b32(src, (int)(4 * lane_off),
+                                                                       4 * (SIDE * (a + 16 * i) + CH * j), 0);
+        }
     }
 }
 
-template <int PREC, int FMT>
+template <int PREC, int FMT, bool REV>
 __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
 {
     typedef typename Elem<FMT>::raw raw;
@@ -160,7 +174,9 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
     const int w0 = wave & 1, w2 = wave >> 1;
     const int g = lane >> 4, c = lane & 15;
     const int t = 16 * w0 + c;                                // the lane's line in a stage
-    const unsigned lane_off = (unsigned)(SIDE * 64 * g + t);  // phase-1 load offset, lane part
+    // phase-1 load offset, lane part (natural order: row 64 g, column t; digit-reversed: word
+    // 4 g of stored row 16 c)
+    const unsigned lane_off = REV ? (unsigned)(4 * g + 16 * SIDE * c) : (unsigned)(SIDE * 64 * g + t);
     // exchange-buffer bases: stage-1 writes for even / odd u, stage-2 reads
     // (the odd base is taken at u = 1 and moved back by u = 1's offset, 64 rows)
     const int exw[2] = {ephys(w2, g, t), ephys(w2 + 4, g, t) - 64 * CH};
@@ -191,7 +207,7 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
     }
     Chunk<PREC, FMT> buf[DEPTH + 1];
 #pragma unroll
-    for (int j = 0; j < DEPTH; ++j) load_chunk<PREC, FMT>(buf[j], src, j, wave, lane_off);
+    for (int j = 0; j < DEPTH; ++j) load_chunk<PREC, FMT, REV>(buf[j], src, j, wave, lane_off);
     if (tab) {
         stw[threadIdx.x] = t0;
         twl[threadIdx.x] = t1;
@@ -206,7 +222,7 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
         if (j + DEPTH < NCH) {
             // issued before this chunk's stage 1 reads its buffer: buf[(j + DEPTH) % (DEPTH
             // + 1)] was consumed by chunk j - 1
-            load_chunk<PREC, FMT>(buf[(j + DEPTH) % (DEPTH + 1)], src, j + DEPTH, wave, lane_off);
+            load_chunk<PREC, FMT, REV>(buf[(j + DEPTH) % (DEPTH + 1)], src, j + DEPTH, wave, lane_off);
         }
         __syncthreads();  // the previous chunk's stage 2 has read the exchange buffer
         stage1<PREC, FMT>(buf[j % (DEPTH + 1)].w, op, stw, ex, w2, g, exw);
@@ -280,12 +296,14 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
 
 }  // namespace onchip
 
-const void *find_onchip_kernel(int prec, int fmt)
+const void *find_onchip_kernel(int prec, int fmt, bool rev)
 {
     using namespace onchip;
-    if (prec == PREC_SPLIT && fmt == FMT_C16) return (const void *)fft_onchip<PREC_SPLIT, FMT_C16>;
-    if (prec == PREC_FP16 && fmt == FMT_C16) return (const void *)fft_onchip<PREC_FP16, FMT_C16>;
-    if (prec == PREC_BF16 && fmt == FMT_CBF16) return (const void *)fft_onchip<PREC_BF16, FMT_CBF16>;
+    if (prec == PREC_SPLIT && fmt == FMT_C16)
+        return rev ? (const void *)fft_onchip<PREC_SPLIT, FMT_C16, true> : (const void *)fft_onchip<PREC_SPLIT, FMT_C16, false>;
+    if (prec == PREC_FP16 && fmt == FMT_C16)
+        return rev ? (const void *)fft_onchip<PREC_FP16, FMT_C16, true> : (const void *)fft_onchip<PREC_FP16, FMT_C16, false>;
+    if (prec == PREC_BF16 && fmt == FMT_CBF16 && !rev) return (const void *)fft_onchip<PREC_BF16, FMT_CBF16, false>;
     return nullptr;
 }
 

@@ -382,20 +382,23 @@ static int plan_axis(std::vector<PassSpec> &out, long long L, long long ps_in, l
     return plan_axis(out, L2, ps_in, L1 * ps_out, sub2, a_dst, dst, -1, 1, narrow);
 }
 
-// 65536-point lines in natural order with 16-bit words in, out and between the two 256-point
-// halves: one pass of the on-chip kernel (the line is read and written once) instead of the
-// four-step pair, unless a split is asked for (split_hint > 1). The halves are the same
-// four-step decomposition (tcfft_half.cu:266-390 then 69-161 for the reference's long lines).
+// 65536-point lines with 16-bit words in, out and between the two 256-point halves, in
+// natural order or in the digit-reversed order of the radix list {16, 16, 16, 16}: one pass
+// of the on-chip kernel (the line is read and written once) instead of the four-step pair,
+// unless a split is asked for (split_hint > 1). The halves are the same four-step
+// decomposition (tcfft_half.cu:266-390 then 69-161 for the reference's long lines).
 static bool plan_onchip(mfft_plan_s *p, const mfft_desc *d)
 {
-    if (p->n != kOnchipN || d->in_order != MFFT_ORDER_NATURAL || d->split_hint > 1) return false;
+    if (p->n != kOnchipN || d->split_hint > 1) return false;
+    const bool rev = d->in_order == MFFT_ORDER_DIGIT_REVERSED;
+    if (rev && p->rev_list_axis[0] != std::vector<int>{16, 16, 16, 16}) return false;
     const int fmt = d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16 : MFFT_FMT_C16;
     const int mid = d->mid_fmt >= 0                         ? d->mid_fmt
                     : d->precision == MFFT_PREC_FP16 ? MFFT_FMT_C16
                     : d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16
                                                      : MFFT_FMT_C32;
     if (d->in_fmt != fmt || d->out_fmt != fmt || mid != fmt) return false;
-    if (!find_onchip_kernel(d->precision, fmt)) return false;
+    if (!find_onchip_kernel(d->precision, fmt, rev)) return false;
     PassSpec ps;
     ps.R = kOnchipN;
     ps.ps_in = ps.ps_out = 1;
@@ -405,6 +408,7 @@ static bool plan_onchip(mfft_plan_s *p, const mfft_desc *d)
     ps.src = BUF_IN;
     ps.dst = BUF_OUT;
     ps.onchip = true;
+    if (rev) ps.rev_list = p->rev_list_axis[0];
     p->passes.push_back(ps);
     return true;
 }
@@ -480,7 +484,8 @@ static int finish_passes(mfft_plan_s *p)
         if (ps.onchip) {
             ps.radix = {16, 16, 16, 16};
             ps.fast_kind = KIND_ONCHIP;
-            ps.fn = find_onchip_kernel(p->d.precision, p->d.in_fmt);
+            ps.fn = find_onchip_kernel(p->d.precision, p->d.in_fmt, !ps.rev_list.empty());
+            ps.rev_fast = !ps.rev_list.empty();  // the kernel reads the stored order itself
             ps.l0 = 0;
             ps.T = 1;
             ps.TL = 1;
@@ -837,7 +842,9 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             return hipLaunchKernel(ps.fn, grid, block, args, 0, s);
         };
         if (ps.fast_kind == KIND_ONCHIP) {
-            // one workgroup per line
+            // one workgroup per line; a digit-reversed line is read in 16-byte pieces
+            if (!ps.rev_list.empty() && ((uintptr_t)a.in & 15))
+                return fail(MFFT_ERR_INVALID, "digit-reversed 65536-point input must be 16-byte aligned");
             OnchipArgs oa;
             oa.in = a.in;
             oa.out = a.out;
@@ -944,10 +951,10 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
         rev_radices(n, p->rev_list_axis[0]);
         gen_rev(n, p->rev_list_axis[0], p->rev_axis[0]);
     }
-    if (rev && n > kMaxTile) {
-        rc = plan_reversed_1d(p);
-    } else if (plan_onchip(p, d)) {
+    if (plan_onchip(p, d)) {
         rc = MFFT_OK;
+    } else if (rev && n > kMaxTile) {
+        rc = plan_reversed_1d(p);
     } else {
         std::vector<Level> lines{Level{batch, n, n, true}};
         rc = plan_axis(p->passes, n, 1, 1, lines, BUF_IN, BUF_OUT, rev ? 0 : -1, d->split_hint, narrow_mid(d));

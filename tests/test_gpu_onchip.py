@@ -4,7 +4,8 @@ natural-order 65536-point lines with 16-bit words in, out and between the halves
 
 Against the FP64 oracle (the reference's check, main.cpp:164-198): split fp16, fp16 and bf16,
 forward and inverse, per-stage and no scaling, batches of 1, 5 and 300 lines (more lines than
-the device runs at once), in place; and against the two-pass plan of the same transform
+the device runs at once), in place, digit-reversed input (tcfftExec's order: gen_rev of
+{16, 16, 16, 16}); and against the two-pass plan of the same transform
 (split_hint = 256: the strided 256-point pass with the four-step twiddle, then the contiguous
 one), which computes the same decomposition in the same arithmetic with the intermediate in
 HBM. Inputs: re/im uniform[-1,1] from SplitMix64, rounded to the input format.
@@ -121,3 +122,29 @@ def test_onchip_impulses(dev):
         print(f"impulse at {n0}: max abs error = {err:.3e}")
         assert err <= 2e-3
     plan.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,scale,batch", [("fp16", 0, 3), ("split_fp16", 0, 4), ("split_fp16", 1, 260)])
+def test_onchip_digit_reversed(dev, prec, scale, batch):
+    """Input stored in the digit-reversed order of four radix-16 digits (gen_rev of
+    {16, 16, 16, 16}, the order tcfftExec reads at 65536 points): the on-chip kernel reads it
+    with one 16-byte load per operand group; natural-order output against the oracle, and
+    bit for bit equal to the natural-order plan on the same (unpermuted) line."""
+    import torch
+
+    plan = _plan(prec, batch, in_order=1, scale=scale)
+    nat = _plan(prec, batch, scale=scale)
+    assert [(p["kernel"], p["R"]) for p in plan.info["passes"]] == [("onchip", N)], plan.info["passes"]
+    rev = oracle.gen_rev(N, [16, 16, 16, 16])
+    assert np.array_equal(plan.input_permutation(), rev)
+    x = complex_input((batch, N), 300 + batch, "c16", -1 / 8, 1 / 8)
+    y = plan.exec(to_device(x[:, rev], plan.in_dtype(), dev))
+    want = nat.exec(to_device(x, nat.in_dtype(), dev))
+    torch.cuda.synchronize()
+    assert torch.equal(y, want)
+    l2, mx = rel_errors(from_device(y) / plan.out_scale, oracle.fft(x))
+    print(f"onchip digit-reversed {prec} scale={scale} b={batch}: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+    plan.close()
+    nat.close()
