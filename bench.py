@@ -46,6 +46,7 @@ sys.path.insert(0, ROOT)
 METRIC = "FFT GFLOP/s (5NlogN) at rel-L2≤1e-3 vs plain FP32 GPU FFT; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 TUNE = True
+SPLIT = None  # --split: the headline plan's first-pass length (None: tuned / planner's)
 PASS_REPS = 200   # whole execs per pass-window measurement (the roofline's launch time)
 PASS_WARM = 20    # execs run pass by pass before it
 # The device's clocks ramp up over the first ~50 ms of continuous work (65536 x 256: pass 0
@@ -235,18 +236,19 @@ def accuracy(plan, xh, out, sample, direction=-1):
     return oracle.err_l2max(yc, ref)
 
 
-def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt=None, passes=True):
+def run_variant(name, shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt=None, passes=True, split=None):
     """Autotune (split point, batch chunk), then time `steps` execs after `warmup`; the
-    per-pass windows when `passes`."""
+    per-pass windows when `passes`. split: this split point instead (no autotuning)."""
     import torch
 
     import fft_demo_amd as F
     from fft_demo_amd import tune
 
     scale = 0 if name == "fp32" else 1  # matrix-core variants scale 1/radix per stage
-    sched = tune.autotune(shape, batch, precision=name, mid_fmt=mid_fmt, scale=scale) if TUNE else None
-    plan = F.Plan(shape, batch, precision=name, split_hint=tune.SPLIT_TUNED if TUNE else 0, mid_fmt=mid_fmt,
-                  scale=scale)
+    tuned = TUNE and split is None
+    sched = tune.autotune(shape, batch, precision=name, mid_fmt=mid_fmt, scale=scale) if tuned else None
+    hint = split if split is not None else (tune.SPLIT_TUNED if tuned else 0)
+    plan = F.Plan(shape, batch, precision=name, split_hint=hint, mid_fmt=mid_fmt, scale=scale)
     x = torch.from_numpy(xh).to(device=dev, dtype=plan.in_dtype()).contiguous()
     out = plan.empty_out(device=dev)
     settle(lambda st: plan.exec(x, out, stream=st), stream)
@@ -519,7 +521,13 @@ def fft_config(cfg, steps, warmup, dev, stream, full=True):
             comps["rocfft_fp32_external"] = rocfft_point(shape, batch, xh, steps, warmup, dev)
         except Exception as e:  # noqa: BLE001
             comps["rocfft_fp32_external"] = {"error": str(e)}
-    head = run_variant("split_fp16", shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt="c16")
+        if shape == (65536,):
+            # the same transform as two HBM passes (strided 256-point pass with the four-step
+            # twiddle, then the contiguous one): the plan the on-chip kernel replaces
+            comps["split_fp16_two_pass"] = run_variant("split_fp16", shape, batch, xh, steps, warmup, dev, stream,
+                                                       sample, mid_fmt="c16", passes=False, split=256)
+    head = run_variant("split_fp16", shape, batch, xh, steps, warmup, dev, stream, sample, mid_fmt="c16",
+                       split=SPLIT)
     return shape, batch, desc, xh, sample, head, comps
 
 
@@ -1005,6 +1013,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="headline workload only (no c1/c3/c4 objects)")
     ap.add_argument("--no-tune", action="store_true", help="planner's schedule, no autotuning (profiling runs)")
+    ap.add_argument("--split", type=int, default=None, metavar="L1",
+                    help="headline plan with this first-pass length instead of the tuned schedule (256 at "
+                         "65536 points: the two-pass plan; profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -1013,8 +1024,9 @@ def main():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 1000 if args.config == "c1" else 200
-    global TUNE
+    global TUNE, SPLIT
     TUNE = not args.no_tune
+    SPLIT = args.split
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched FFT shards along "
               f"the batch with no exchange, so the per-GPU figure is the 1-GPU figure.", file=sys.stderr)
