@@ -45,6 +45,7 @@ def _load():
     lib.oracle_gen_rev.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip]
     lib.oracle_gen_rev.restype = ctypes.c_int
     lib.oracle_fill_uniform.argtypes = [_dp, _i64, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
+    lib.oracle_splitmix64.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64), _i64]
     lib.oracle_round_fp16.argtypes = [_dp, _i64]
     lib.oracle_round_bf16.argtypes = [_dp, _i64]
     lib.oracle_dft.argtypes = [_dp, _dp, _i64, _i64, ctypes.c_int]
@@ -90,6 +91,14 @@ def gen_rev(n: int, radices) -> np.ndarray:
     rc = _load().oracle_gen_rev(int(n), rad.ctypes.data_as(_ip), len(rad), out.ctypes.data_as(_ip))
     if rc != 0:
         raise ValueError(f"radices {list(radices)} do not multiply to {n}")
+    return out
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """The raw SplitMix64 stream (64-bit outputs) the uniform generator draws from."""
+    out = np.empty(int(count), dtype=np.uint64)
+    _load().oracle_splitmix64(ctypes.c_uint64(seed & (2**64 - 1)), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                              int(count))
     return out
 
 

@@ -65,6 +65,12 @@ static uint64_t splitmix64(uint64_t *s)
     return z ^ (z >> 31);
 }
 
+void oracle_splitmix64(uint64_t seed, uint64_t *out, int64_t count)
+{
+    uint64_t s = seed;
+    for (int64_t i = 0; i < count; ++i) out[i] = splitmix64(&s);
+}
+
 void oracle_fill_uniform(double *dst, int64_t count, uint64_t seed, double lo, double hi)
 {
     uint64_t s = seed;

@@ -29,6 +29,10 @@ extern "C" {
  * -1 if the radices do not multiply to n. */
 int oracle_gen_rev(int n, const int *radices, int n_radices, int *rev);
 
+/* The raw SplitMix64 stream (Steele, Lea and Flood, "Fast splittable pseudorandom number
+ * generators", OOPSLA 2014; Vigna's reference splitmix64.c): count outputs from `seed`. */
+void oracle_splitmix64(uint64_t seed, uint64_t *out, int64_t count);
+
 /* SplitMix64 uniform[-1,1] generator (seeded stream). x = splitmix64(state);
  * u = (x >> 11) * 2^-53; value = lo + (hi - lo) * u. */
 void oracle_fill_uniform(double *dst, int64_t count, uint64_t seed, double lo, double hi);

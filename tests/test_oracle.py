@@ -80,6 +80,18 @@ def test_bf16_rounding_ties_to_even():
     assert np.array_equal(oracle.round_bf16(yv), yv)
 
 
+def test_splitmix64_published_vector():
+    """The generator behind every synthetic input: SplitMix64 (Steele, Lea and Flood, OOPSLA
+    2014; Vigna's reference splitmix64.c), whose published output for seed 1234567 begins
+    6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+    16408922859458223821; the uniform values are (x >> 11) * 2^-53 of that stream."""
+    want = [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+            16408922859458223821]
+    assert [int(v) for v in oracle.splitmix64(1234567, 5)] == want
+    u = oracle.uniform(5, 1234567, 0.0, 1.0)
+    assert [float(v) for v in u] == [(w >> 11) * 2.0 ** -53 for w in want]
+
+
 def test_uniform_stream_range_and_seed():
     a = oracle.uniform(100000, 42)
     assert a.min() >= -1 and a.max() < 1 and abs(a.mean()) < 0.01
