@@ -108,6 +108,10 @@ struct TileArgs {
     // four-step twiddle of a digit-reversed first pass: the exponent's line factor is
     // digit_rev(line, tw_rev) (identity when tw_rev.n = 0)
     DigitRev tw_rev;
+    // cos(pi/8), sin(pi/8), cos(pi/4): the constants of the VALU radix-8 and radix-16
+    // butterflies, generated in FP64 on the GPU and rounded once to fp32 like every other
+    // table (gen_w16)
+    float w16[3];
 };
 
 // DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of

@@ -549,7 +549,7 @@ __device__ __forceinline__ void dft4v(float2 *v, int sign)
     v[3] = make_float2(d0.x - r.x, d0.y - r.y);
 }
 
-__device__ __forceinline__ void dft16v(float2 *v, int sign)
+__device__ __forceinline__ void dft16v(float2 *v, int sign, const float *w16)
 {
     float2 a[4][4];
 #pragma unroll
@@ -559,7 +559,7 @@ __device__ __forceinline__ void dft16v(float2 *v, int sign)
 #pragma unroll
         for (int k1 = 0; k1 < 4; ++k1) a[k1][j2] = t[k1];
     }
-    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+    const float c1 = w16[0], s1 = w16[1], h = w16[2];
     const float wr[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
     const float wi[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
 #pragma unroll
@@ -597,7 +597,7 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
         for (int j = 0; j < 16; ++j) v[j] = lds[G::phys(t, col + j * R16)];
         if constexpr (!LAST) __syncthreads();
     }
-    dft16v(v, a.sign);
+    dft16v(v, a.sign, a.w16);
     if constexpr (!LAST) {
         const int q = col % s, p = col / s;
 #pragma unroll
@@ -633,7 +633,7 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
 
 // ------------------------------------------------------------------ tail stage (radix 2/4/8, VALU)
 template <int RADIX>
-__device__ __forceinline__ void dft_tail(float2 *v, int sign)
+__device__ __forceinline__ void dft_tail(float2 *v, int sign, const float *w16)
 {
     if constexpr (RADIX == 2) {
         float2 x0 = v[0], x1 = v[1];
@@ -642,7 +642,7 @@ __device__ __forceinline__ void dft_tail(float2 *v, int sign)
     } else if constexpr (RADIX == 4) {
         dft4v(v, sign);
     } else {
-        const float h = 0.70710678118654752440f;
+        const float h = w16[2];
         float2 a4[4], b4[4];
 #pragma unroll
         for (int j2 = 0; j2 < 4; ++j2) {
@@ -692,7 +692,7 @@ __device__ __forceinline__ void tstage(float2 *lds, const TileArgs &a, int s, fl
         if (ITEMS % NT != 0 && I >= ITEMS) break;
         const int t = KIND == KIND_COLS ? I % T : I / COLS;
         const int col = KIND == KIND_COLS ? I / T : I % COLS;
-        dft_tail<RADIX>(v[k], a.sign);
+        dft_tail<RADIX>(v[k], a.sign, a.w16);
         const int q = col % s, p = col / s;
         const float comp = mul / a.last_mul;
 #pragma unroll
@@ -731,7 +731,7 @@ __device__ __forceinline__ void tstage_last(float2 *lds, const TileArgs &a, cons
 #pragma unroll
         for (int j = 0; j < RADIX; ++j) v[j] = lds[G::phys(t, col + j * COLS)];
         if (t >= o.nv) continue;
-        dft_tail<RADIX>(v, a.sign);
+        dft_tail<RADIX>(v, a.sign, a.w16);
         raw *dst = reinterpret_cast<raw *>(a.out) + o.out_base + (long long)t * a.out_s0;
         const unsigned mult = tw_line<REV>(a, o, t);
 #pragma unroll

@@ -29,6 +29,7 @@ namespace mfft {
 hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s,
                        double scale = 1.0);
 hipError_t gen_dft_ops(DftOps *ops, int sign, hipStream_t s);
+hipError_t gen_w16(float *host3);
 typedef void (*tile_fn)(TileArgs);
 tile_fn find_tile_kernel(int prec, int R, int TL, bool generic);
 int pow2_tile_lines(int R, int TL);
@@ -619,6 +620,19 @@ static int finish_passes(mfft_plan_s *p)
     return MFFT_OK;
 }
 
+// The VALU butterflies' constants (TileArgs::w16), generated on the GPU once per process by
+// the first plan or batcher.
+static float g_w16[3];
+static std::once_flag g_w16_once;
+static hipError_t g_w16_err = hipSuccess;
+
+static int ensure_w16()
+{
+    std::call_once(g_w16_once, [] { g_w16_err = gen_w16(g_w16); });
+    if (g_w16_err != hipSuccess) return fail(MFFT_ERR_HIP, "generating the butterfly constants: %s", hipGetErrorString(g_w16_err));
+    return MFFT_OK;
+}
+
 static int alloc_dev(mfft_plan_s *p, void **ptr, size_t bytes)
 {
     hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
@@ -629,6 +643,7 @@ static int alloc_dev(mfft_plan_s *p, void **ptr, size_t bytes)
 
 static int build_tables(mfft_plan_s *p)
 {
+    if (ensure_w16()) return MFFT_ERR_HIP;
     int sign = p->d.direction;
     if (alloc_dev(p, (void **)&p->ops, sizeof(DftOps))) return MFFT_ERR_NOMEM;
     HIP_TRY(gen_dft_ops(p->ops, sign, 0));
@@ -831,6 +846,7 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         a.rev_input = ps.rev_fast ? 1 : 0;
         a.in_rev = ps.rev_q;
         a.tw_rev = ps.rev_tw;
+        memcpy(a.w16, g_w16, sizeof a.w16);
         long long ntiles = ((a.c0 + a.T - 1) / a.T) * (long long)a.c1 * a.c2;
         if (ntiles <= 0) continue;
         if (ntiles >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
@@ -1168,6 +1184,7 @@ int mfft_batcher_create(mfft_batcher *out, const mfft_desc *d, int max_items)
     if (!out || !d || max_items < 1) return fail(MFFT_ERR_INVALID, "bad argument");
     int rc = check_desc(d);
     if (rc) return rc;
+    if (ensure_w16()) return MFFT_ERR_HIP;
     mfft_batcher b = new mfft_batcher_s;
     b->d = *d;
     b->max_items = max_items;
@@ -1247,6 +1264,7 @@ int mfft_batcher_exec(mfft_batcher b, const mfft_batch_item *items, int nitems, 
         a.sign = b->d.direction;
         a.in_vec = ((uintptr_t)it.in & 15) == 0;
         a.out_vec = ((uintptr_t)it.out & 15) == 0;
+        memcpy(a.w16, g_w16, sizeof a.w16);
         sl.h_first[i] = total;
         total += (it.count + a.T - 1) / a.T;
     }

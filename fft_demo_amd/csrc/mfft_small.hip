@@ -54,9 +54,10 @@ __device__ __forceinline__ void dft4(float2 *v, int sg)
     v[3] = make_float2(d0.x - r.x, d0.y - r.y);
 }
 
-// DFT of R points in place, fp32, exact small-angle constants for W_R.
+// DFT of R points in place, fp32; w16 = cos(pi/8), sin(pi/8), cos(pi/4) (TileArgs::w16),
+// used by R = 8 and 16.
 template <int R>
-__device__ __forceinline__ void dft(float2 *v, int sg)
+__device__ __forceinline__ void dft(float2 *v, int sg, const float *w16)
 {
     if constexpr (R == 2) {
         float2 a = v[0], b = v[1];
@@ -73,7 +74,7 @@ __device__ __forceinline__ void dft(float2 *v, int sg)
             a[j] = make_float2(v[j].x + v[j + H].x, v[j].y + v[j + H].y);
             b[j] = make_float2(v[j].x - v[j + H].x, v[j].y - v[j + H].y);
         }
-        const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+        const float c1 = w16[0], s1 = w16[1], h = w16[2];
         // W_16^e (cos, sin) for e = 0..7; W_8^e = W_16^(2e)
         const float wc[8] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1};
         const float ws[8] = {0.f, s1, h, c1, 1.f, c1, h, s1};
@@ -82,8 +83,8 @@ __device__ __forceinline__ void dft(float2 *v, int sg)
             const int e = j * (16 / R);
             b[j] = cm(b[j], make_float2(wc[e], sg * ws[e]));
         }
-        dft<H>(a, sg);
-        dft<H>(b, sg);
+        dft<H>(a, sg, w16);
+        dft<H>(b, sg, w16);
 #pragma unroll
         for (int k = 0; k < H; ++k) {
             v[2 * k] = a[k];
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
     if constexpr (R & 1)
         dft_odd<R>(v, a.stage_tw);
     else
-        dft<R>(v, a.sign);
+        dft<R>(v, a.sign, a.w16);
     const unsigned mult = (unsigned)(a.tw_level ? l1 : l0);
 #pragma unroll
     for (int k = 0; k < R; ++k) {

@@ -49,6 +49,30 @@ __global__ void gen_dft_ops_kernel(DftOps *ops, int sign)
     }
 }
 
+// cos(pi/8), sin(pi/8), cos(pi/4) in FP64, rounded once to fp32: the constants of the VALU
+// radix-8 and radix-16 butterflies (TileArgs::w16).
+__global__ void gen_w16_kernel(float *out)
+{
+    if (threadIdx.x != 0) return;
+    double sn, cs;
+    sincospi(1.0 / 8.0, &sn, &cs);
+    out[0] = (float)cs;
+    out[1] = (float)sn;
+    out[2] = (float)cospi(1.0 / 4.0);
+}
+
+hipError_t gen_w16(float *host3)
+{
+    float *d = nullptr;
+    hipError_t e = hipMalloc(&d, 3 * sizeof(float));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(gen_w16_kernel, dim3(1), dim3(64), 0, 0, d);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(host3, d, 3 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e;
+}
+
 hipError_t gen_twiddle(float2 *out, long long n, long long count, long long mult, int sign, hipStream_t s,
                        double scale)
 {

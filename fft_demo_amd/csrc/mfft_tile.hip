@@ -236,8 +236,9 @@ __device__ void stage16_mfma(float2 *lds, const Geo &G, int s, float mul, const 
 }
 
 // ------------------------------------------------------------------ VALU stages
+// w16: cos(pi/8), sin(pi/8), cos(pi/4) (TileArgs::w16; radix 8 and 16 only)
 template <int RADIX>
-__device__ __forceinline__ void dft_small(float2 *v, int sign)
+__device__ __forceinline__ void dft_small(float2 *v, int sign, const float *w16 = nullptr)
 {
     if constexpr (RADIX == 2) {
         float2 a = v[0], b = v[1];
@@ -256,7 +257,7 @@ __device__ __forceinline__ void dft_small(float2 *v, int sign)
         v[3] = make_float2(d0.x - r.x, d0.y - r.y);
     } else if constexpr (RADIX == 8) {
         // 8 = 2 x 4: x[j], j = 4*j1 + j2 ; X[k1 + 2*k2]
-        const float h = 0.70710678118654752440f;
+        const float h = w16[2];
         float2 a[4], b[4];
 #pragma unroll
         for (int j2 = 0; j2 < 4; ++j2) {
@@ -285,8 +286,8 @@ __device__ __forceinline__ void dft_small(float2 *v, int sign)
 #pragma unroll
             for (int k1 = 0; k1 < 4; ++k1) a[k1][j2] = t[k1];
         }
-        const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
-        const float h = 0.70710678118654752440f;
+        const float c1 = w16[0], s1 = w16[1];
+        const float h = w16[2];
         // W_16^{j2*k1}: (k1,j2) in {1,2,3}^2
         const float wr[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
         const float wi[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
@@ -357,7 +358,7 @@ __device__ void stage_valu(float2 *lds, const Geo &G, int s, float mul, const Ti
         int t = b * TL + tl;
         int q = col % s, p = col / s;
         if constexpr (RADIX == 2 || RADIX == 4 || RADIX == 8 || RADIX == 16)
-            dft_small<RADIX>(v[k], a.sign);
+            dft_small<RADIX>(v[k], a.sign, a.w16);
         else
             dft_odd<RADIX>(v[k], a.stage_tw, cols);
 #pragma unroll
