@@ -45,7 +45,7 @@ class Desc(ctypes.Structure):
     ]
 
 
-KERNEL_NAMES = {0: "tile_pow2", 1: "tile_generic", 2: "fast_rows", 3: "fast_rows_tr", 4: "fast_cols", 6: "onchip",
+KERNEL_NAMES = {0: "tile_pow2", 1: "tile_generic", 2: "fast_rows", 3: "fast_rows_tr", 4: "fast_cols", 6: "onchip", 7: "onchip_cols",
                 5: "stream_small"}
 
 

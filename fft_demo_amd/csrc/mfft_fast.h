@@ -21,7 +21,7 @@
 
 namespace mfft {
 
-enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2, KIND_ONCHIP = 3 /* mfft_onchip.h */ };
+enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2, KIND_ONCHIP = 3, KIND_ONCHIP_COLS = 4 /* mfft_onchip.h */ };
 // Lines per tile of the strided kind: R * T = 8192 points at every length (128 lines at 64
 // points, 64 at 128, 32 at 256, 16 at 512, 8 at 1024). A 16-wave workgroup then keeps two
 // 16-column MFMA groups per wave, whose accumulators, stage values and next-tile operands fit

@@ -296,6 +296,187 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
 
 }  // namespace onchip
 
+// ------------------------------------------------------------------ strided 2048-point columns
+// 32 adjacent columns of 2048 points per workgroup (the column axis of a 2048-row 2D
+// transform in one pass; the reference's 2D column kernels, tcfft_half_2d.cu:265-381, take
+// these lengths in several strided merges). n = n_lo + 128 n_hi:
+//   phase 1: for each (column, n_lo): DFT16 over n_hi (one MFMA stage, exact fp16 inputs),
+//            times W_2048^(n_lo k_hi) -> Y[k_hi][n_lo][column];
+//   phase 2: for each (column, k_hi): DFT128 over n_lo: a radix-16 MFMA stage over the high
+//            digit (n_lo = a + 8 b), times W_128^(a k_b), then a radix-8 VALU stage over a
+//            -> X[k_hi + 16 (k_b + 16 k_a)].
+// As in the 65536-point kernel the intermediate stays on chip: four rounds of 4 k_hi values,
+// round 0 written to LDS by phase 1, rounds 1-3 held in registers (96 words per lane) and
+// dumped into LDS at the round's start. Phase 1 needs no exchange buffer and no barrier.
+namespace onchip_cols {
+using fast::f4;
+using fast::h8;
+using fast::b8;
+using fast::Ops;
+using fast::dft16;
+using fast::cmulf;
+using fast::Elem;
+
+constexpr int NT = 512;
+constexpr int LC = kOnchipColsLines;  // 32 columns
+constexpr int M = 128;                // n = n_lo + 128 n_hi
+constexpr int NCH = 8;                // phase-1 chunks of 16 n_lo values
+constexpr int GPW = 4;
+
+// Y word (khl, n_lo, column) of a round (k_hi = 4 r + khl): [khl][n_lo][column], the column
+// XOR-swizzled by 16 s, s = bit 0 of (khl ^ n_lo >> 5): phase-1 writes (16 consecutive columns
+// x 4 khl) and phase-2 reads (16 consecutive columns x 4 n_lo 32 apart) hit 32 distinct words
+// per half wave.
+__host__ __device__ constexpr int ycol(int khl, int nlo, int col)
+{
+    return (khl * M + nlo) * LC + (col ^ (((khl ^ (nlo >> 5)) & 1) << 4));
+}
+// Exchange buffer (phase 2, two khl per block): float2 at [khl & 1][a][k_b][column], the column
+// XOR-swizzled by bit 0 of k_b: stage-A writes (16 columns x 4 consecutive k_b) and stage-B
+// reads (32 consecutive columns at one k_b) are both conflict-free.
+__host__ __device__ constexpr int ecol(int kh2, int a, int kb, int col)
+{
+    return ((kh2 * 8 + a) * 16 + kb) * LC + (col ^ ((kb & 1) << 4));
+}
+
+template <int PREC, int FMT>
+__global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
+{
+    typedef typename Elem<FMT>::raw raw;
+    static_assert(sizeof(raw) == 4, "16-bit complex words");
+    __shared__ raw ylds[4 * M * LC];     // 64 KB: the rows of one round
+    __shared__ float2 ex[2 * 8 * 16 * LC];  // 64 KB: phase-2 exchange (two khl)
+    __shared__ float2 twl[kOnchipColsN];    // 16 KB: W_2048 (scaled)
+    __shared__ float2 stw[M];               // W_128 (scaled)
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w0 = wave & 1, w2 = wave >> 1;
+    const int g = lane >> 4, c = lane & 15;
+    const int col = 16 * w0 + c;  // the lane's column in the MFMA stages
+
+    Ops<PREC> op;
+    const int m = lane & 15;  // permuted operand rows: output row 4g + i holds frequency 4i + g
+    op.load(reinterpret_cast<const DftOps *>(a.dft_ops), (lane & 48) | (4 * (m & 3) + (m >> 2)));
+
+    const long long img = blockIdx.x / a.col_blocks;
+    const int cb = (int)(blockIdx.x - img * a.col_blocks);
+    const raw *src = reinterpret_cast<const raw *>(a.in) + img * a.in_dist + (long long)LC * cb;
+    raw *dst = reinterpret_cast<raw *>(a.out) + img * a.out_dist + (long long)LC * cb;
+    // one resource per column block: 2048 rows of the pitch (the block's last column ends
+    // 32 words after the last row's start)
+    const int span = 4 * ((kOnchipColsN - 1) * a.ps + LC);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<raw *>(src), 0, span, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, 0, span, 0x00020000);
+
+    // phase-1 loads of chunk j, group u: column col, n_lo = 16 j + w2 + 4 u, n_hi = 4 g + i at
+    // word col + (n_lo + 128 (4 g + i)) ps: lane part col + 512 g ps, uniform part (n_lo + 128 i) ps
+    const int lane_ld = 4 * (col + 4 * M * g * a.ps);
+    auto load_chunk = [&](raw *w, int j) {
+#pragma unroll
+        for (int u = 0; u < GPW; ++u)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                w[4 * u + i] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane_ld, 4 * (16 * j + w2 + 4 * u + M * i) * a.ps, 0);
+    };
+    raw buf[2][GPW * 4];
+    // the tables are read before the first chunk is requested and written to LDS after it
+    float2 tt[kOnchipColsN / NT];
+#pragma unroll
+    for (int k = 0; k < kOnchipColsN / NT; ++k) tt[k] = a.tw_s[threadIdx.x + NT * k];
+    float2 ts = make_float2(0.f, 0.f);
+    if (threadIdx.x < M) ts = a.stw_s[threadIdx.x];
+    load_chunk(buf[0], 0);
+#pragma unroll
+    for (int k = 0; k < kOnchipColsN / NT; ++k) twl[threadIdx.x + NT * k] = tt[k];
+    if (threadIdx.x < M) stw[threadIdx.x] = ts;
+    __syncthreads();
+
+    raw hold[NCH][GPW][3];
+    // ---------------- phase 1: one radix-16 stage, twiddle, round 0 to LDS, rounds 1-3 held
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+        if (j + 1 < NCH) load_chunk(buf[(j + 1) & 1], j + 1);
+        const raw *w = buf[j & 1];
+#pragma unroll
+        for (int u = 0; u < GPW; ++u) {
+            fast::u4 v = {w[4 * u], w[4 * u + 1], w[4 * u + 2], w[4 * u + 3]};
+            const float2 x[4] = {};
+            f4 ar, ai;
+            dft16<PREC, true>(x, __builtin_bit_cast(h8, v), __builtin_bit_cast(b8, v), op, ar, ai);
+            const int nlo = 16 * j + w2 + 4 * u;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // k_hi = 4 i + g: round i, khl = g
+                const float2 y = cmulf(make_float2(ar[i], ai[i]), twl[nlo * (4 * i + g)]);
+                if (i == 0)
+                    ylds[ycol(g, nlo, col)] = Elem<FMT>::pack(y);
+                else
+                    hold[j][u][i - 1] = Elem<FMT>::pack(y);
+            }
+        }
+    }
+
+    // ---------------- phase 2: four rounds (k_hi = 4 r + khl), two blocks of two khl each
+    const int tcol = threadIdx.x & 31, tkb = (threadIdx.x >> 5) & 15;  // stage-B thread map
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (r > 0) {
+            // the previous round's stage-A reads of these words ended at the barrier before its
+            // last exchange write
+#pragma unroll
+            for (int j = 0; j < NCH; ++j)
+#pragma unroll
+                for (int u = 0; u < GPW; ++u) ylds[ycol(g, 16 * j + w2 + 4 * u, col)] = hold[j][u][r - 1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int hb = 0; hb < 2; ++hb) {
+            // stage A: group u -> (khl = 2 hb + (p >> 3), a = p & 7), p = w2 + 4 u; lane inputs
+            // b = 4 g + i (n_lo = a + 8 b) of column col
+            f4 ar[GPW], ai[GPW];
+#pragma unroll
+            for (int u = 0; u < GPW; ++u) {
+                const int p = w2 + 4 * u, khl = 2 * hb + (p >> 3), aa = p & 7;
+                fast::u4 v;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = ylds[ycol(khl, aa + 8 * (4 * g + i), col)];
+                const float2 x[4] = {};
+                dft16<PREC, true>(x, __builtin_bit_cast(h8, v), __builtin_bit_cast(b8, v), op, ar[u], ai[u]);
+            }
+            __syncthreads();  // the previous block's stage B has read the exchange buffer
+#pragma unroll
+            for (int u = 0; u < GPW; ++u) {
+                const int p = w2 + 4 * u, kh2 = p >> 3, aa = p & 7;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kb = 4 * i + g;
+                    ex[ecol(kh2, aa, kb, col)] = cmulf(make_float2(ar[u][i], ai[u][i]), stw[aa * kb]);
+                }
+            }
+            __syncthreads();
+            // stage B: thread -> (column tcol, k_b = tkb), both khl of the block; radix 8 over a,
+            // stored at k = k_hi + 16 (k_b + 16 k_a): 32 consecutive columns per row
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                float2 v[8];
+#pragma unroll
+                for (int aa = 0; aa < 8; ++aa) v[aa] = ex[ecol(q, aa, tkb, tcol)];
+                fast::dft_tail<8>(v, a.sign, a.w16);
+                const int khi = 4 * r + 2 * hb + q;
+#pragma unroll
+                for (int ka = 0; ka < 8; ++ka) {
+                    const int k = khi + 16 * (tkb + 16 * ka);
+                    const float2 y = make_float2(v[ka].x * a.tail_mul, v[ka].y * a.tail_mul);
+                    __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(y), rd, 4 * tcol, 4 * k * a.ps, 0);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace onchip_cols
+
 const void *find_onchip_kernel(int prec, int fmt, bool rev)
 {
     using namespace onchip;
@@ -304,6 +485,15 @@ const void *find_onchip_kernel(int prec, int fmt, bool rev)
     if (prec == PREC_FP16 && fmt == FMT_C16)
         return rev ? (const void *)fft_onchip<PREC_FP16, FMT_C16, true> : (const void *)fft_onchip<PREC_FP16, FMT_C16, false>;
     if (prec == PREC_BF16 && fmt == FMT_CBF16 && !rev) return (const void *)fft_onchip<PREC_BF16, FMT_CBF16, false>;
+    return nullptr;
+}
+
+const void *find_onchip_cols_kernel(int prec, int fmt)
+{
+    using namespace onchip_cols;
+    if (prec == PREC_SPLIT && fmt == FMT_C16) return (const void *)fft_onchip_cols<PREC_SPLIT, FMT_C16>;
+    if (prec == PREC_FP16 && fmt == FMT_C16) return (const void *)fft_onchip_cols<PREC_FP16, FMT_C16>;
+    if (prec == PREC_BF16 && fmt == FMT_CBF16) return (const void *)fft_onchip_cols<PREC_BF16, FMT_CBF16>;
     return nullptr;
 }
 

@@ -214,6 +214,7 @@ struct PassSpec {
     bool rev_fast = false;  // the fast kernel reads the digit-reversed lines (rev_q)
     DigitRev rev_q{};
     bool onchip = false;    // the whole 65536-point line in one pass (fft_onchip)
+    bool onchip_cols = false;  // 2048-point strided columns, 32 per workgroup (fft_onchip_cols)
 };
 
 struct mfft_plan_s {
@@ -234,7 +235,32 @@ struct mfft_plan_s {
     std::vector<int> rev_list_axis[2];  // the radix lists rev_axis was generated from
     std::mutex exec_mu;       // the workspace is per plan
     StreamOrder ws_order;     // streams that used the plan's own workspace
+    // A plan whose passes include an on-chip kernel (one workgroup per line, or per 32
+    // columns) keeps the plan of the same transform without it for execs of fewer than
+    // small_below transforms, which would leave most of the device idle on the on-chip pass.
+    mfft_plan_s *small = nullptr;
+    long long small_below = 0;
 };
+
+// Transforms (lines, or images of 2048-row 2D plans) an on-chip pass needs to occupy the
+// device enough to beat the multi-pass plan: a workgroup per 65536-point line, 2/3 of the
+// compute units (65536 x 256 on 256 CUs: 36 us against 53 us; at 128 lines the two-pass plan is
+// faster); a workgroup per 32 columns of 2048 points, 3/4 of them (2048 x 2048: one image
+// 49 us against 38 us, eight 129 us against 149 us).
+static long long device_cus()
+{
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || ncu < 1)
+        ncu = 256;
+    return ncu;
+}
+static long long onchip_min_lines() { return (2 * device_cus() + 2) / 3; }
+static long long onchip_cols_min_images(int ny)
+{
+    const long long blocks = std::max(1, ny / kOnchipColsLines);
+    return std::max(1LL, (3 * device_cus() / 4 + blocks - 1) / blocks);
+}
 
 static int fmt_bytes(int fmt) { return fmt == FMT_C32 ? 8 : 4; }
 
@@ -482,6 +508,16 @@ static int plan_reversed_1d(mfft_plan_s *p)
 static int finish_passes(mfft_plan_s *p)
 {
     for (auto &ps : p->passes) {
+        if (ps.onchip_cols) {
+            ps.radix = {16, 16, 8};
+            ps.fast_kind = KIND_ONCHIP_COLS;
+            ps.fn = find_onchip_cols_kernel(p->d.precision, p->mid_fmt);
+            ps.l0 = 0;
+            ps.T = kOnchipColsLines;
+            ps.TL = 1;
+            if (!ps.fn) return fail(MFFT_ERR_UNSUPPORTED, "no on-chip column kernel for the formats");
+            continue;
+        }
         if (ps.onchip) {
             ps.radix = {16, 16, 16, 16};
             ps.fast_kind = KIND_ONCHIP;
@@ -648,8 +684,9 @@ static int build_tables(mfft_plan_s *p)
     if (alloc_dev(p, (void **)&p->ops, sizeof(DftOps))) return MFFT_ERR_NOMEM;
     HIP_TRY(gen_dft_ops(p->ops, sign, 0));
     for (auto &ps : p->passes) {
-        // the on-chip kernel's stages are 256-point halves
-        const int Rst = ps.onchip ? 256 : ps.R;
+        // the on-chip kernel's stages are 256-point halves; the column kernel's second half
+        // is 128 points (its 2048-point twiddle table is built below)
+        const int Rst = ps.onchip ? 256 : ps.onchip_cols ? 128 : ps.R;
         if (alloc_dev(p, (void **)&ps.stage_tw, sizeof(float2) * Rst)) return MFFT_ERR_NOMEM;
         HIP_TRY(gen_twiddle(ps.stage_tw, Rst, Rst, 1, sign, 0));
         if (ps.tw_N) {
@@ -678,6 +715,11 @@ static int build_tables(mfft_plan_s *p)
                 if (alloc_dev(p, (void **)&ps.tw_lo_s, sizeof(float2) * nlo)) return MFFT_ERR_NOMEM;
                 HIP_TRY(gen_twiddle(ps.tw_lo_s, ps.tw_N, nlo, 1, sign, 0, fac));
             }
+        }
+        if (ps.onchip_cols) {
+            // W_2048^e with the radix-16 stage scale, e in [0, 2048)
+            if (alloc_dev(p, (void **)&ps.tw_lo_s, sizeof(float2) * kOnchipColsN)) return MFFT_ERR_NOMEM;
+            HIP_TRY(gen_twiddle(ps.tw_lo_s, kOnchipColsN, kOnchipColsN, 1, sign, 0, ps.last_mul));
         }
         if (ps.small) ps.last_mul = p->d.scale == MFFT_SCALE_STAGE ? 1.0f / ps.R : 1.0f;
         if (ps.perm_axis >= 0 && p->d.in_order == MFFT_ORDER_DIGIT_REVERSED && !ps.rev_fast) {
@@ -752,6 +794,7 @@ static void trace_pass(const mfft_plan_s *p, const PassSpec &ps, const TileArgs 
                       : ps.fast_kind == KIND_ROWS_TR ? "fast_rows_tr"
                       : ps.fast_kind == KIND_COLS ? "fast_cols"
                       : ps.fast_kind == KIND_ONCHIP ? "onchip"
+                      : ps.fast_kind == KIND_ONCHIP_COLS ? "onchip_cols"
                       : ps.generic ? "tile_generic" : "tile";
     const long long pts = (long long)a.c0 * a.c1 * a.c2 * ps.R;
     long long rd = pts * fmt_bytes(a.in_fmt);
@@ -857,6 +900,32 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             if (kstart || kstop) return hipExtLaunchKernel(ps.fn, grid, block, args, 0, s, kstart, kstop, 0);
             return hipLaunchKernel(ps.fn, grid, block, args, 0, s);
         };
+        if (ps.fast_kind == KIND_ONCHIP_COLS) {
+            // one workgroup per 32 adjacent columns of one image
+            OnchipColsArgs oa;
+            memset(&oa, 0, sizeof oa);
+            oa.in = a.in;
+            oa.out = a.out;
+            oa.col_blocks = (int)(ps.lines[0].count / kOnchipColsLines);
+            oa.blocks = (long long)oa.col_blocks * nb;
+            oa.ps = (int)ps.ps_in;
+            oa.in_dist = ps.lines[1].in_stride;
+            oa.out_dist = ps.lines[1].out_stride;
+            oa.dft_ops = p->ops;
+            oa.tw_s = ps.tw_lo_s;
+            oa.stw_s = ps.stage_tw_s;
+            oa.tail_mul = ps.tail_mul;
+            oa.sign = p->d.direction;
+            memcpy(oa.w16, g_w16, sizeof oa.w16);
+            void *oargs[] = {&oa};
+            if (oa.blocks >= (1LL << 31)) return fail(MFFT_ERR_UNSUPPORTED, "grid too large");
+            const dim3 grid((unsigned)oa.blocks), block(kOnchipThreads);
+            if (kstart || kstop)
+                HIP_TRY(hipExtLaunchKernel(ps.fn, grid, block, oargs, 0, s, kstart, kstop, 0));
+            else
+                HIP_TRY(hipLaunchKernel(ps.fn, grid, block, oargs, 0, s));
+            continue;
+        }
         if (ps.fast_kind == KIND_ONCHIP) {
             // one workgroup per line; a digit-reversed line is read in 16-byte pieces
             if (!ps.rev_list.empty() && ((uintptr_t)a.in & 15))
@@ -947,7 +1016,7 @@ static mfft_desc resolve_tuned(const mfft_desc *d, int nx, int ny, int64_t batch
     return r;
 }
 
-int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d_in)
+static int create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d_in, bool allow_onchip)
 {
     if (!plan || !d_in) return fail(MFFT_ERR_INVALID, "null argument");
     *plan = nullptr;
@@ -967,7 +1036,7 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
         rev_radices(n, p->rev_list_axis[0]);
         gen_rev(n, p->rev_list_axis[0], p->rev_axis[0]);
     }
-    if (plan_onchip(p, d)) {
+    if (allow_onchip && batch >= onchip_min_lines() && plan_onchip(p, d)) {
         rc = MFFT_OK;
     } else if (rev && n > kMaxTile) {
         rc = plan_reversed_1d(p);
@@ -987,7 +1056,33 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     return MFFT_OK;
 }
 
-int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in)
+static bool has_onchip(const mfft_plan_s *p)
+{
+    for (auto &ps : p->passes)
+        if (ps.onchip || ps.onchip_cols) return true;
+    return false;
+}
+
+int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d_in)
+{
+    int rc = create_1d(plan, n, batch, d_in, true);
+    if (rc || !has_onchip(*plan)) return rc;
+    mfft_plan p = *plan, q = nullptr;
+    const long long below = onchip_min_lines();
+    rc = create_1d(&q, n, below - 1, d_in, false);
+    if (rc) {
+        std::string msg = g_err;
+        mfft_plan_destroy(p);
+        *plan = nullptr;
+        g_err = msg;
+        return rc;
+    }
+    p->small = q;
+    p->small_below = below;
+    return MFFT_OK;
+}
+
+static int create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in, bool allow_onchip)
 {
     if (!plan || !d_in) return fail(MFFT_ERR_INVALID, "null argument");
     *plan = nullptr;
@@ -1029,8 +1124,25 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
         // split_hint > 1: the first-pass length of the column axis when it is split
         const int col_hint = d->split_hint > 1 ? d->split_hint : odd_hint;
         std::vector<Level> cols{Level{ny, 1, 1, false}, Level{batch, NN, NN, true}};
-        rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, col_hint, narrow_mid(d),
-                       rev_cols);
+        // 2048-point columns with 16-bit words on both sides, a multiple of 32 of them: one
+        // pass of the on-chip column kernel (the split into a strided and a streaming pass
+        // moves every point twice)
+        const int fmt16 = d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16 : MFFT_FMT_C16;
+        if (allow_onchip && nx == kOnchipColsN && ny % kOnchipColsLines == 0 && batch >= onchip_cols_min_images(ny) &&
+            !rev && d->split_hint <= 1 && col_in_fmt == fmt16 && d->out_fmt == fmt16 &&
+            find_onchip_cols_kernel(d->precision, fmt16)) {
+            PassSpec ps;
+            ps.R = nx;
+            ps.ps_in = ps.ps_out = ny;
+            ps.lines = cols;
+            ps.src = BUF_WORK;
+            ps.dst = BUF_OUT;
+            ps.onchip_cols = true;
+            p->passes.push_back(ps);
+        } else {
+            rc = plan_axis(p->passes, nx, ny, ny, cols, BUF_WORK, BUF_OUT, rev ? 0 : -1, col_hint, narrow_mid(d),
+                           rev_cols);
+        }
     }
     if (!rc) rc = finalize_plan(p);
     if (rc) {
@@ -1044,10 +1156,30 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
     return MFFT_OK;
 }
 
+int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in)
+{
+    int rc = create_2d(plan, nx, ny, batch, d_in, true);
+    if (rc || !has_onchip(*plan)) return rc;
+    mfft_plan p = *plan, q = nullptr;
+    const long long below = onchip_cols_min_images(ny);
+    rc = create_2d(&q, nx, ny, below - 1, d_in, false);
+    if (rc) {
+        std::string msg = g_err;
+        mfft_plan_destroy(p);
+        *plan = nullptr;
+        g_err = msg;
+        return rc;
+    }
+    p->small = q;
+    p->small_below = below;
+    return MFFT_OK;
+}
+
 int mfft_plan_destroy(mfft_plan plan)
 {
     if (!plan) return MFFT_OK;
     (void)hipDeviceSynchronize();
+    if (plan->small) mfft_plan_destroy(plan->small);
     for (void *ptr : plan->allocs) (void)hipFree(ptr);
     plan->ws_order.destroy();
     delete plan;
@@ -1089,10 +1221,12 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
                     : ps.fast_kind == KIND_ROWS_TR ? MFFT_KERNEL_ROWS_TR
                     : ps.fast_kind == KIND_COLS ? MFFT_KERNEL_COLS
                     : ps.fast_kind == KIND_ONCHIP ? MFFT_KERNEL_ONCHIP
+                    : ps.fast_kind == KIND_ONCHIP_COLS ? MFFT_KERNEL_ONCHIP_COLS
                     : ps.generic ? MFFT_KERNEL_GENERIC : MFFT_KERNEL_TILE;
     }
     info->out_scale = p->out_scale;
-    info->workspace_bytes = p->work_bytes;
+    // a caller-supplied workspace also serves the companion plan of small execs
+    info->workspace_bytes = std::max(p->work_bytes, p->small ? p->small->work_bytes : 0LL);
     info->batch_chunk = p->chunk;
     info->launches_per_exec = nchunks * (long long)p->passes.size();
     return MFFT_OK;
@@ -1102,6 +1236,9 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
                     void *work = nullptr, const mfft::ExecFuse *fz = nullptr, int only = -1,
                     hipEvent_t kstart = nullptr, hipEvent_t kstop = nullptr)
 {
+    if (p->small && batch >= 0 && batch < p->small_below && only < 0 && !ev &&
+        (!fz || (p->small->passes.front().small && p->small->passes.front().src == BUF_IN)))
+        return exec_all(p->small, in, out, stream, nullptr, batch, work, fz, -1, nullptr, nullptr);
     std::unique_lock<std::mutex> lk(p->exec_mu, std::defer_lock);
     if (!work) lk.lock();  // the plan's own workspace is shared by its callers
     hipStream_t s = (hipStream_t)stream;

@@ -99,7 +99,8 @@ enum {
     MFFT_KERNEL_ROWS_TR = 3,   /* matrix-core kernel, contiguous in, transposed out (fft_fast) */
     MFFT_KERNEL_COLS = 4,      /* matrix-core kernel, strided 256-point lines (fft_fast) */
     MFFT_KERNEL_STREAM = 5,    /* streaming short-DFT kernel, one thread per line (small_cols) */
-    MFFT_KERNEL_ONCHIP = 6     /* one workgroup per 65536-point line, the intermediate on chip (fft_onchip) */
+    MFFT_KERNEL_ONCHIP = 6,    /* one workgroup per 65536-point line, the intermediate on chip (fft_onchip) */
+    MFFT_KERNEL_ONCHIP_COLS = 7 /* 32 strided 2048-point columns per workgroup, on chip (fft_onchip_cols) */
 };
 
 typedef struct mfft_info {

@@ -84,7 +84,7 @@ def test_c2_split_c16_mid_full_batch(dev, schedule):
 def test_c2_split_c16_mid_inverse(dev, split):
     import fft_demo_amd as F
 
-    n, batch = 65536, 16
+    n, batch = 65536, 256
     plan = F.Plan((n,), batch, precision="split_fp16", mid_fmt="c16", direction=1, split_hint=split)
     want = [("onchip", n)] if split == 0 else [("fast_cols", 256), ("fast_rows_tr", 256)]
     assert [k[:2] for k in _kinds(plan)] == want

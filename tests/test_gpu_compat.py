@@ -171,8 +171,8 @@ def test_digit_reversed_plans_on_fast_kernels(dev, prec, n, batch):
     """MFFT_ORDER_DIGIT_REVERSED plans (tcfftExec's input order, gen_rev of the reference's
     radix list) run on the matrix-core kernels, which read the reversed order in their
     first-stage loads: one pass up to 4096 points, two contiguous-line passes above (the
-    first with its four-step twiddle on the digit-reversed line index), and at 65536 points
-    with 16-bit intermediates (fp16) the one-pass on-chip kernel. Unnormalised, as
+    first with its four-step twiddle on the digit-reversed line index; with many 65536-point
+    lines the one-pass on-chip kernel, tests/test_gpu_onchip.py). Unnormalised, as
     tcfftExec; tolerance fp16 rel-L2 4e-3 / max 2e-2, split fp16 1e-3 / 4e-3."""
     import fft_demo_amd as F
 
@@ -180,11 +180,8 @@ def test_digit_reversed_plans_on_fast_kernels(dev, prec, n, batch):
     plan = F.Plan((n,), batch, precision=prec, in_order=1, scale=0)
     kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
     print(f"n={n} {prec}: passes {kinds}")
-    if n == 65536 and prec == "fp16":
-        assert kinds == [("onchip", n)], kinds
-    else:
-        assert all(k in ("fast_rows", "fast_rows_tr") for k, _ in kinds), kinds
-        assert len(kinds) == (1 if n <= 4096 else 2)
+    assert all(k in ("fast_rows", "fast_rows_tr") for k, _ in kinds), kinds
+    assert len(kinds) == (1 if n <= 4096 else 2)
     rev = oracle.gen_rev(n, REF_RADICES[n])
     assert np.array_equal(plan.input_permutation(), rev)
     x = complex_input((batch, n), 200 + n % 89)

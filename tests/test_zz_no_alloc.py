@@ -24,11 +24,12 @@ def _noalloc(lib, plan, x, out, batch, stream):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape,batch,small,mid,split", [((65536,), 32, 5, "c16", 256), ((65536,), 32, 5, "c16", 0),
+@pytest.mark.parametrize("shape,batch,small,mid,split", [((65536,), 32, 5, "c16", 256), ((65536,), 256, 5, "c16", 0),
                                                          ((1024, 1024), 8, 3, None, 0), ((4096,), 64, 17, None, 0)])
 def test_exec_noalloc_matches_exec(dev, shape, batch, small, mid, split):
     """split = 256 at 65536 points: the two-pass plan (intermediates in the scratch array);
-    split = 0 there: the one-pass on-chip plan, which has no intermediates in memory."""
+    256 lines with split = 0: the one-pass on-chip plan, which has no intermediates in memory,
+    and whose exec of 5 lines runs its companion two-pass plan (intermediates in the array)."""
     import torch
 
     import fft_demo_amd as F
