@@ -21,16 +21,20 @@ struct OnchipArgs {
 constexpr int kOnchipN = 65536;
 constexpr int kOnchipThreads = 512;
 
-// Strided 2048-point lines (the column axis of a 2D transform with 2048 rows): one
-// workgroup transforms 32 adjacent columns whole, in place in the same layout (column c,
-// point n at c + n * ps). 16-bit words on both sides.
+// 2048-point lines, 32 per workgroup, transformed whole, 16-bit words on both sides:
+//   columns: the column axis of a 2D transform with 2048 rows, in place in the same layout
+//            (column c, point n at c + n * ps on both sides);
+//   rows:    32 consecutive contiguous lines (line l, point n at 2048 l + n), stored
+//            transposed (point k of line l at l + k * ps_out): the second pass of a 1D
+//            four-step plan N = L1 x 2048 (ps_out = L1).
 struct OnchipColsArgs {
     const void *in;
     void *out;
-    long long blocks;              // workgroups: (columns / 32) per image x images
-    int col_blocks;                // 32-column blocks per image
-    int ps;                        // words between consecutive points of a column (the row pitch)
-    long long in_dist, out_dist;   // words between consecutive images
+    long long blocks;              // workgroups: 32-line blocks per transform x transforms
+    int col_blocks;                // 32-line blocks per transform
+    int ps;                        // columns: words between consecutive points (the row pitch)
+    int ps_out;                    // rows: words between consecutive output points (L1)
+    long long in_dist, out_dist;   // words between consecutive transforms
     const void *dft_ops;           // DftOps of the transform's direction
     const float2 *tw_s;            // W_2048^e * fac, e in [0, 2048)
     const float2 *stw_s;           // W_128^e * fac, e in [0, 128)
@@ -42,7 +46,7 @@ struct OnchipColsArgs {
 constexpr int kOnchipColsN = 2048;
 constexpr int kOnchipColsLines = 32;
 
-const void *find_onchip_cols_kernel(int prec, int fmt);
+const void *find_onchip_cols_kernel(int prec, int fmt, bool rows = false);
 
 // The kernel for a precision and its 16-bit format, or null. rev: the lines are stored in the
 // digit-reversed order of four radix-16 digits (split fp16 and fp16 only).

@@ -331,6 +331,18 @@ __host__ __device__ constexpr int ycol(int khl, int nlo, int col)
 {
     return (khl * M + nlo) * LC + (col ^ (((khl ^ (nlo >> 5)) & 1) << 4));
 }
+// The rows kind keeps Y as [khl][line][n_lo] with n_lo XOR-swizzled by (line & 15) | 16 (khl & 1):
+// its phase-1 writes walk 16 consecutive n_lo (x 4 khl), its phase-2 reads 16 consecutive
+// lines (x 4 n_lo 32 apart), both on 32 distinct words per half wave.
+template <bool ROWS>
+__host__ __device__ constexpr int yaddr(int khl, int nlo, int line)
+{
+    if constexpr (ROWS)
+        return (khl * LC + line) * M + (nlo ^ ((line & 15) | ((khl & 1) << 4)));
+    else
+        return ycol(khl, nlo, line);
+}
+
 // Exchange buffer (phase 2, two khl per block): float2 at [khl & 1][a][k_b][column], the column
 // XOR-swizzled by bit 0 of k_b: stage-A writes (16 columns x 4 consecutive k_b) and stage-B
 // reads (32 consecutive columns at one k_b) are both conflict-free.
@@ -339,7 +351,7 @@ __host__ __device__ constexpr int ecol(int kh2, int a, int kb, int col)
     return ((kh2 * 8 + a) * 16 + kb) * LC + (col ^ ((kb & 1) << 4));
 }
 
-template <int PREC, int FMT>
+template <int PREC, int FMT, bool ROWS>
 __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
 {
     typedef typename Elem<FMT>::raw raw;
@@ -361,23 +373,36 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
 
     const long long img = blockIdx.x / a.col_blocks;
     const int cb = (int)(blockIdx.x - img * a.col_blocks);
-    const raw *src = reinterpret_cast<const raw *>(a.in) + img * a.in_dist + (long long)LC * cb;
+    // the block's first word: columns cb * 32 of the image; rows 32 cb of the transform (input)
+    // and, stored transposed, word 32 cb (output)
+    const raw *src = reinterpret_cast<const raw *>(a.in) + img * a.in_dist + (long long)LC * cb * (ROWS ? kOnchipColsN : 1);
     raw *dst = reinterpret_cast<raw *>(a.out) + img * a.out_dist + (long long)LC * cb;
-    // one resource per column block: 2048 rows of the pitch (the block's last column ends
-    // 32 words after the last row's start)
-    const int span = 4 * ((kOnchipColsN - 1) * a.ps + LC);
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<raw *>(src), 0, span, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, 0, span, 0x00020000);
+    // one resource per block: 2048 rows of the pitch (the last line ends 32 words after the
+    // last row's start); the rows kind reads 32 contiguous lines
+    const int ps_out = ROWS ? a.ps_out : a.ps;
+    const int span_in = ROWS ? 4 * LC * kOnchipColsN : 4 * ((kOnchipColsN - 1) * a.ps + LC);
+    const int span_out = 4 * ((kOnchipColsN - 1) * ps_out + LC);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<raw *>(src), 0, span_in, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, 0, span_out, 0x00020000);
 
-    // phase-1 loads of chunk j, group u: column col, n_lo = 16 j + w2 + 4 u, n_hi = 4 g + i at
-    // word col + (n_lo + 128 (4 g + i)) ps: lane part col + 512 g ps, uniform part (n_lo + 128 i) ps
-    const int lane_ld = 4 * (col + 4 * M * g * a.ps);
+    // phase-1 group u of chunk j and the lane's (line, n_lo); the lane loads n_hi = 4 g + i.
+    //   columns: line = col, n_lo = 16 j + w2 + 4 u; word line + (n_lo + 128 (4 g + i)) ps: lane
+    //            part col + 512 g ps, uniform part (n_lo + 128 i) ps (16 lanes: 64 bytes of a row);
+    //   rows:    line = 4 j + u, n_lo = 16 wave + c; word 2048 line + n_lo + 128 (4 g + i): lane
+    //            part c + 512 g, uniform part 2048 line + 16 wave + 128 i (a chunk reads 4 whole
+    //            lines)
+    auto p1_line = [&](int j, int u) { return ROWS ? 4 * j + u : col; };
+    auto p1_nlo = [&](int j, int u) { return ROWS ? 16 * wave + c : 16 * j + w2 + 4 * u; };
+    const int lane_ld = ROWS ? 4 * (c + 4 * M * g) : 4 * (col + 4 * M * g * a.ps);
     auto load_chunk = [&](raw *w, int j) {
 #pragma unroll
         for (int u = 0; u < GPW; ++u)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                w[4 * u + i] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane_ld, 4 * (16 * j + w2 + 4 * u + M * i) * a.ps, 0);
+            for (int i = 0; i < 4; ++i) {
+                const int uni = ROWS ? 4 * (kOnchipColsN * (4 * j + u) + 16 * wave + M * i)
+                                     : 4 * (16 * j + w2 + 4 * u + M * i) * a.ps;
+                w[4 * u + i] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane_ld, uni, 0);
+            }
     };
     raw buf[2][GPW * 4];
     // the tables are read before the first chunk is requested and written to LDS after it
@@ -404,13 +429,13 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
             const float2 x[4] = {};
             f4 ar, ai;
             dft16<PREC, true>(x, __builtin_bit_cast(h8, v), __builtin_bit_cast(b8, v), op, ar, ai);
-            const int nlo = 16 * j + w2 + 4 * u;
+            const int nlo = p1_nlo(j, u);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 // k_hi = 4 i + g: round i, khl = g
                 const float2 y = cmulf(make_float2(ar[i], ai[i]), twl[nlo * (4 * i + g)]);
                 if (i == 0)
-                    ylds[ycol(g, nlo, col)] = Elem<FMT>::pack(y);
+                    ylds[yaddr<ROWS>(g, nlo, p1_line(j, u))] = Elem<FMT>::pack(y);
                 else
                     hold[j][u][i - 1] = Elem<FMT>::pack(y);
             }
@@ -427,7 +452,7 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
 #pragma unroll
             for (int j = 0; j < NCH; ++j)
 #pragma unroll
-                for (int u = 0; u < GPW; ++u) ylds[ycol(g, 16 * j + w2 + 4 * u, col)] = hold[j][u][r - 1];
+                for (int u = 0; u < GPW; ++u) ylds[yaddr<ROWS>(g, p1_nlo(j, u), p1_line(j, u))] = hold[j][u][r - 1];
         }
         __syncthreads();
 #pragma unroll
@@ -440,7 +465,7 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
                 const int p = w2 + 4 * u, khl = 2 * hb + (p >> 3), aa = p & 7;
                 fast::u4 v;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = ylds[ycol(khl, aa + 8 * (4 * g + i), col)];
+                for (int i = 0; i < 4; ++i) v[i] = ylds[yaddr<ROWS>(khl, aa + 8 * (4 * g + i), col)];
                 const float2 x[4] = {};
                 dft16<PREC, true>(x, __builtin_bit_cast(h8, v), __builtin_bit_cast(b8, v), op, ar[u], ai[u]);
             }
@@ -455,8 +480,8 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
                 }
             }
             __syncthreads();
-            // stage B: thread -> (column tcol, k_b = tkb), both khl of the block; radix 8 over a,
-            // stored at k = k_hi + 16 (k_b + 16 k_a): 32 consecutive columns per row
+            // stage B: thread -> (line tcol, k_b = tkb), both khl of the block; radix 8 over a,
+            // stored at k = k_hi + 16 (k_b + 16 k_a): 32 consecutive lines per output row
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 float2 v[8];
@@ -468,7 +493,7 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
                 for (int ka = 0; ka < 8; ++ka) {
                     const int k = khi + 16 * (tkb + 16 * ka);
                     const float2 y = make_float2(v[ka].x * a.tail_mul, v[ka].y * a.tail_mul);
-                    __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(y), rd, 4 * tcol, 4 * k * a.ps, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(y), rd, 4 * tcol, 4 * k * ps_out, 0);
                 }
             }
         }
@@ -488,12 +513,18 @@ const void *find_onchip_kernel(int prec, int fmt, bool rev)
     return nullptr;
 }
 
-const void *find_onchip_cols_kernel(int prec, int fmt)
+const void *find_onchip_cols_kernel(int prec, int fmt, bool rows)
 {
     using namespace onchip_cols;
-    if (prec == PREC_SPLIT && fmt == FMT_C16) return (const void *)fft_onchip_cols<PREC_SPLIT, FMT_C16>;
-    if (prec == PREC_FP16 && fmt == FMT_C16) return (const void *)fft_onchip_cols<PREC_FP16, FMT_C16>;
-    if (prec == PREC_BF16 && fmt == FMT_CBF16) return (const void *)fft_onchip_cols<PREC_BF16, FMT_CBF16>;
+    if (prec == PREC_SPLIT && fmt == FMT_C16)
+        return rows ? (const void *)fft_onchip_cols<PREC_SPLIT, FMT_C16, true>
+                    : (const void *)fft_onchip_cols<PREC_SPLIT, FMT_C16, false>;
+    if (prec == PREC_FP16 && fmt == FMT_C16)
+        return rows ? (const void *)fft_onchip_cols<PREC_FP16, FMT_C16, true>
+                    : (const void *)fft_onchip_cols<PREC_FP16, FMT_C16, false>;
+    if (prec == PREC_BF16 && fmt == FMT_CBF16)
+        return rows ? (const void *)fft_onchip_cols<PREC_BF16, FMT_CBF16, true>
+                    : (const void *)fft_onchip_cols<PREC_BF16, FMT_CBF16, false>;
     return nullptr;
 }
 

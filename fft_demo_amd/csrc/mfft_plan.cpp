@@ -214,7 +214,8 @@ struct PassSpec {
     bool rev_fast = false;  // the fast kernel reads the digit-reversed lines (rev_q)
     DigitRev rev_q{};
     bool onchip = false;    // the whole 65536-point line in one pass (fft_onchip)
-    bool onchip_cols = false;  // 2048-point strided columns, 32 per workgroup (fft_onchip_cols)
+    bool onchip_cols = false;  // 2048-point lines, 32 per workgroup (fft_onchip_cols)
+    bool onchip_rows = false;  // ... contiguous lines stored transposed (its rows kind)
 };
 
 struct mfft_plan_s {
@@ -256,6 +257,11 @@ static long long device_cus()
     return ncu;
 }
 static long long onchip_min_lines() { return (2 * device_cus() + 2) / 3; }
+static long long onchip_rows_min_lines(int l1)
+{
+    const long long blocks = std::max(1, l1 / kOnchipColsLines);
+    return std::max(1LL, (3 * device_cus() / 4 + blocks - 1) / blocks);
+}
 static long long onchip_cols_min_images(int ny)
 {
     const long long blocks = std::max(1, ny / kOnchipColsLines);
@@ -440,6 +446,48 @@ static bool plan_onchip(mfft_plan_s *p, const mfft_desc *d)
     return true;
 }
 
+// N = L1 x 2048 (2^19, 2^20) with 16-bit words in, out and between the passes: the four-step
+// split with the 2048-point half on the on-chip line kernel -- pass A: DFT_L1 over n1 (stride
+// 2048, the strided matrix-core kernel) times W_N^(n2 k1); pass B: DFT_2048 of each row k1, 32
+// rows per workgroup read whole and stored transposed at k1 + L1 k2. Split fp16 on MI355X:
+// 2^20 x 16 / 64 in 72 / 303 us against 78 / 316 us for the three-pass plan, 2^19 x 32 / 64 in
+// 63 / 131 us against 68 / 172 us for 1024 + 512; at 2^17 and 2^18 (L1 = 64, 128) it was no
+// faster than the 1024-point first pass, which those lengths keep.
+static bool plan_onchip_rows(mfft_plan_s *p, const mfft_desc *d)
+{
+    const long long N = p->n;
+    if (N < (1LL << 19) || N > (1LL << 20) || !is_pow2(N) || d->in_order != MFFT_ORDER_NATURAL || d->split_hint > 1)
+        return false;
+    const int L1 = (int)(N / kOnchipColsN);
+    const int fmt = d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16 : MFFT_FMT_C16;
+    const int mid = d->mid_fmt >= 0                         ? d->mid_fmt
+                    : d->precision == MFFT_PREC_FP16 ? MFFT_FMT_C16
+                    : d->precision == MFFT_PREC_BF16 ? MFFT_FMT_CBF16
+                                                     : MFFT_FMT_C32;
+    if (d->in_fmt != fmt || d->out_fmt != fmt || mid != fmt) return false;
+    if (!find_onchip_cols_kernel(d->precision, fmt, true) || !find_fast_kernel(d->precision, L1, KIND_COLS, fmt, fmt))
+        return false;
+    PassSpec a;
+    a.R = L1;
+    a.ps_in = a.ps_out = kOnchipColsN;
+    a.lines = {Level{p->batch, N, N, true}, Level{kOnchipColsN, 1, 1, false}};
+    a.tw_N = (int)N;
+    a.tw_level = 1;
+    a.src = BUF_IN;
+    a.dst = BUF_WORK;
+    PassSpec b;
+    b.R = kOnchipColsN;
+    b.ps_in = 1;
+    b.ps_out = L1;
+    b.lines = {Level{L1, kOnchipColsN, 1, false}, Level{p->batch, N, N, true}};
+    b.src = BUF_WORK;
+    b.dst = BUF_OUT;
+    b.onchip_cols = b.onchip_rows = true;
+    p->passes.push_back(a);
+    p->passes.push_back(b);
+    return true;
+}
+
 // Digit-reversed input (tcfftExec's order, tcfft_half.cu:1177-1233) of a 1D length longer
 // than one tile: N = L1 L2, split after the first m radices of the list. Position
 // p = p_lo + L1 p_hi holds natural element n2 + L2 n1 with n1 = gen_rev(list[:m])[p_lo] and
@@ -511,7 +559,7 @@ static int finish_passes(mfft_plan_s *p)
         if (ps.onchip_cols) {
             ps.radix = {16, 16, 8};
             ps.fast_kind = KIND_ONCHIP_COLS;
-            ps.fn = find_onchip_cols_kernel(p->d.precision, p->mid_fmt);
+            ps.fn = find_onchip_cols_kernel(p->d.precision, p->mid_fmt, ps.onchip_rows);
             ps.l0 = 0;
             ps.T = kOnchipColsLines;
             ps.TL = 1;
@@ -909,6 +957,7 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
             oa.col_blocks = (int)(ps.lines[0].count / kOnchipColsLines);
             oa.blocks = (long long)oa.col_blocks * nb;
             oa.ps = (int)ps.ps_in;
+            oa.ps_out = (int)ps.ps_out;
             oa.in_dist = ps.lines[1].in_stride;
             oa.out_dist = ps.lines[1].out_stride;
             oa.dft_ops = p->ops;
@@ -1038,6 +1087,9 @@ static int create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *d_i
     }
     if (allow_onchip && batch >= onchip_min_lines() && plan_onchip(p, d)) {
         rc = MFFT_OK;
+    } else if (allow_onchip && n >= (1 << 19) && n <= (1 << 20) &&
+               batch >= onchip_rows_min_lines(n / kOnchipColsN) && plan_onchip_rows(p, d)) {
+        rc = MFFT_OK;
     } else if (rev && n > kMaxTile) {
         rc = plan_reversed_1d(p);
     } else {
@@ -1068,7 +1120,7 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     int rc = create_1d(plan, n, batch, d_in, true);
     if (rc || !has_onchip(*plan)) return rc;
     mfft_plan p = *plan, q = nullptr;
-    const long long below = onchip_min_lines();
+    const long long below = n == kOnchipN ? onchip_min_lines() : onchip_rows_min_lines(n / kOnchipColsN);
     rc = create_1d(&q, n, below - 1, d_in, false);
     if (rc) {
         std::string msg = g_err;

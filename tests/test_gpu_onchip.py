@@ -263,3 +263,64 @@ def test_onchip_cols_impulses(dev):
         print(f"impulse at ({n0}, {c0}): max abs error = {err:.3e}")
         assert err <= 2e-3
     plan.close()
+
+
+# ---------------------------------------------------------------- 1D N = L1 x 2048
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,n,batch,direction", [("split_fp16", 1 << 20, 16, -1), ("split_fp16", 1 << 19, 32, 1),
+                                                    ("split_fp16", 1 << 20, 13, 1), ("split_fp16", 1 << 19, 64, -1),
+                                                    ("fp16", 1 << 20, 12, -1), ("bf16", 1 << 19, 24, -1)])
+def test_onchip_rows_1d_vs_oracle(dev, prec, n, batch, direction):
+    """1D lengths L1 x 2048 (2^19, 2^20) with 16-bit words and enough 32-row blocks to occupy
+    3/4 of the compute units: pass A the strided L1-point matrix-core pass with the four-step
+    twiddle, pass B the on-chip line kernel's rows kind (32 whole rows per workgroup, stored
+    transposed); against the oracle."""
+    import fft_demo_amd as F
+
+    kw = dict(mid_fmt="c16") if prec == "split_fp16" else {}
+    plan = F.Plan((n,), batch, precision=prec, direction=direction, **kw)
+    kinds = [(p["kernel"], p["R"]) for p in plan.info["passes"]]
+    assert kinds == [("fast_cols", n // 2048), ("onchip_cols", 2048)], kinds
+    x = complex_input((batch, n), 1100 + n % 997 + batch, FMT[prec])
+    y = from_device(plan.exec(to_device(x, plan.in_dtype(), dev))) / plan.out_scale
+    l2, mx = rel_errors(y, oracle.fft(x, sign=direction))
+    print(f"onchip rows {prec} n={n} b={batch} dir={direction}: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+    plan.close()
+
+
+@pytest.mark.gpu
+def test_onchip_rows_matches_multipass_and_small_execs(dev):
+    """2^20 x 16: the two-pass plan with the on-chip rows agrees with the three-pass plan
+    (split_hint = 128) well within either's error bound; an exec of 3 lines on it runs its
+    companion multi-pass plan (bit for bit equal to a plan of 3 lines)."""
+    import ctypes
+
+    import torch
+
+    import fft_demo_amd as F
+    from fft_demo_amd import _lib as L
+
+    n, b = 1 << 20, 16
+    one = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16")
+    three = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16", split_hint=128)
+    few = F.Plan((n,), 3, precision="split_fp16", mid_fmt="c16")
+    assert [p["kernel"] for p in one.info["passes"]][-1] == "onchip_cols"
+    assert "onchip_cols" not in [p["kernel"] for p in three.info["passes"] + few.info["passes"]]
+    x = complex_input((b, n), 1201)
+    xd = to_device(x, one.in_dtype(), dev)
+    y1 = from_device(one.exec(xd))
+    y3 = from_device(three.exec(xd))
+    l2, mx = rel_errors(y1, y3)
+    print(f"2^20: two passes (on-chip rows) vs three: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= 1e-3 and mx <= 4e-3
+    want = few.exec(xd[:3].contiguous())
+    got = torch.zeros_like(want)
+    s = torch.cuda.current_stream(dev)
+    lib = L.load()
+    L.check(lib.mfft_exec_ex(one._h, ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(got.data_ptr()), 3, None,
+                             ctypes.c_void_p(s.cuda_stream)), "mfft_exec_ex")
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    for p in (one, three, few):
+        p.close()
