@@ -60,6 +60,9 @@ CONFIGS = {
     "c2": ((65536,), 256, "batched 1D complex FFT N=65536 (16^4) x 256, fp16 I/O"),
     "c1": ((256,), 4096, "batched 1D complex FFT N=256 x 4096, fp16 I/O"),
     "c3": ((1024, 1024), 64, "batched 2D complex FFT 1024x1024 x 64, 10% of entries x100, fp16 I/O"),
+    # the serving mix's two largest FFT shapes (--config only; not in the default line)
+    "m2d2048": ((2048, 2048), 16, "batched 2D complex FFT 2048x2048 x 16, fp16 I/O"),
+    "m1d2p20": ((1 << 20,), 64, "batched 1D complex FFT N=2^20 x 64, fp16 I/O"),
 }
 
 
@@ -281,8 +284,13 @@ def roofline_of(res, key_prefix=""):
     achieved = algo / (pp[k] * 1e-3) / 1e9
     tkey = f"{p['kernel']}:R{p['R']}:c{p['in_elem_bytes'] * 4}>c{p['out_elem_bytes'] * 4}"
     tmeas = profiled_traffic().get(key_prefix + tkey)
+    traffic = None
+    if tmeas:
+        # an int is bytes per launch of the profiled workload; a float is traffic / algorithmic
+        # bytes, which carries over to other batch chunks of the same kernel and format
+        traffic = int(tmeas[0] * algo) if isinstance(tmeas[0], float) else tmeas[0]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tmeas[0] if tmeas else None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": f"profiles/{tmeas[1]} ({key_prefix + tkey})" if tmeas else None,
             "kernel": f"pass {k}: {p['kernel']} (R={p['R']}, lines/tile={p['lines_per_tile']}, "
                       f"c{p['in_elem_bytes'] * 4} in / c{p['out_elem_bytes'] * 4} out)",
@@ -391,9 +399,9 @@ def copy_bandwidth(mib=1024, reps=10):
 def profiled_traffic():
     """HBM traffic measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of
     known size), from the summaries committed under profiles/: {key: (bytes per launch, file)}
-    for key = "<kernel family>:R<points per line>:c<in bits>>c<out bits>", and
-    {"conv:" + launch-trace name: (traffic / algorithmic bytes, file)} for the conv launches,
-    whose sizes depend on the batch chunk."""
+    or {key: (traffic / algorithmic bytes, file)} for key = "<kernel family>:R<points per
+    line>:c<in bits>>c<out bits>", and {"conv:" + launch-trace name: (traffic / algorithmic
+    bytes, file)} for the conv launches, whose sizes depend on the batch chunk."""
     out = {}
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
