@@ -654,10 +654,11 @@ def conv_trace(op, x, out, stream, reps=20):
 def conv_config(steps, warmup, with_cpu=True):
     """FFT 'same' convolution: 2048x2048 fp32 images U[0,1], batch 16, one 31x31 filter
     (Gaussian sigma 5 normalised to sum 1, times 1 + U[-0.01, 0.01]), padded to 2176; the
-    filter spectrum resident (fp32). Variants: split-fp16 transforms with fp32 image
-    spectra (the configuration's precision), the same with fp16 image spectra and
-    intermediates (half the bytes, rel-L2 ~5e-4), plain fp32 transforms, rocFFT real
-    transforms (external). Accuracy on rows of two images against the direct FP64
+    filter spectrum resident (fp32). Variants: split-fp16 transforms with fp32 image spectra
+    (the configuration's spectra are split fp16) and fp16 words at the pass boundaries inside
+    each transform (the headline, as the 1D and 2D workloads run), the same with fp32 pass
+    boundaries, fp16 image spectra and intermediates (half the bytes, rel-L2 ~5e-4), plain
+    fp32 transforms, rocFFT real transforms (external). Accuracy on rows of two images against the direct FP64
     convolution."""
     import numpy as np
     import torch
@@ -698,15 +699,17 @@ def conv_config(steps, warmup, with_cpu=True):
 
     res = {}
     traces = {}
-    for name, prec, spec16 in (("fp32", "fp32", False), ("split_fp16_fp16_spectra", "split_fp16", True),
-                               ("split_fp16", "split_fp16", False)):
-        op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision=prec, spec_fp16=spec16)
+    for name, prec, kw in (("fp32", "fp32", {}), ("split_fp16_fp16_spectra", "split_fp16", {"spec_fp16": True}),
+                           ("split_fp16", "split_fp16", {"mid_fp16": True}),
+                           ("split_fp16_fp32_boundaries", "split_fp16", {})):
+        op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision=prec, **kw)
         op.set_filters(gd)
         ms = timed(lambda: op.forward(x, out))
         l2, mx = acc()
         traces[name] = conv_trace(op, x, out, stream)
         res[name] = {"images_per_s": round(B / (ms * 1e-3), 1), "ms_per_batch": round(ms, 4), "rel_l2": l2,
-                     "rel_max": mx, "pad": list(op.pad), "image_spectra": "fp16" if spec16 else "fp32"}
+                     "rel_max": mx, "pad": list(op.pad), "image_spectra": "fp16" if kw.get("spec_fp16") else "fp32",
+                     "pass_boundaries": "fp16" if kw else "fp32"}
         op.close()
     # labelled external point: torch.fft real transforms (rocFFT), resident filter spectrum
     p0, p1 = res["split_fp16"]["pad"]
@@ -756,20 +759,18 @@ def conv_config(steps, warmup, with_cpu=True):
     head = res["split_fp16"]
     P = head["pad"][0] * head["pad"][1]
     outd = {"workload": "FFT 'same' conv2d, 2048x2048 fp32 x 16, 31x31 Gaussian filter, pad 2176, split-fp16 "
-                        "transforms, fp32 image spectra, resident fp32 filter spectrum",
+                        "transforms, fp32 image spectra, fp16 words at the pass boundaries inside each transform "
+                        "(as the 1D and 2D workloads), resident fp32 filter spectrum",
             "value": head["images_per_s"], "unit": "images/s", "ms_per_batch": head["ms_per_batch"],
             "steps": steps, "nominal_gflops": round(B * 2 * 5.0 * P * math.log2(P) / (head["ms_per_batch"] * 1e-3) / 1e9, 1),
             "pad": head["pad"], "rel_l2": head["rel_l2"], "rel_max": head["rel_max"],
             "accuracy_sample": f"rows {rows[0]}..{rows[1] - 1} of images 0 and {B - 1} vs direct FP64 convolution",
             "roofline": trace_summary(traces["split_fp16"], head["ms_per_batch"]),
             "ratio_vs_plain_fp32": round(head["images_per_s"] / res["fp32"]["images_per_s"], 3),
-            "comparators": {"split_fp16_fp16_spectra": dict(res["split_fp16_fp16_spectra"],
-                                                            roofline=trace_summary(traces["split_fp16_fp16_spectra"],
-                                                                                   res["split_fp16_fp16_spectra"]["ms_per_batch"]),
-                                                            ratio_vs_plain_fp32=round(
-                                                                res["split_fp16_fp16_spectra"]["images_per_s"] /
-                                                                res["fp32"]["images_per_s"], 3)),
-                            "fp32": res["fp32"], "rocfft_fp32_external": res["rocfft_fp32_external"]}}
+            "comparators": dict({k: dict(res[k], roofline=trace_summary(traces[k], res[k]["ms_per_batch"]),
+                                         ratio_vs_plain_fp32=round(res[k]["images_per_s"] / res["fp32"]["images_per_s"], 3))
+                                 for k in ("split_fp16_fp32_boundaries", "split_fp16_fp16_spectra")},
+                                fp32=res["fp32"], rocfft_fp32_external=res["rocfft_fp32_external"])}
     if with_cpu:
         # direct FP64 'same' convolution (961 MAC per pixel) on 64 rows of one image, 1 thread
         r = (0, 64)

@@ -15,7 +15,10 @@ MODES = {"same": L.CONV_SAME, "valid_corr": L.CONV_VALID_CORR}
 
 class Conv2d:
     def __init__(self, batch, channels, filters, h, w, fh, fw, mode="same", pad=None, precision="split_fp16",
-                 workspace_limit=0, spec_fp16=False):
+                 workspace_limit=0, spec_fp16=False, mid_fp16=False):
+        """spec_fp16: fp16 image spectra and intermediates; mid_fp16: fp32 image spectra with
+        fp16 pass boundaries inside each transform (one channel, one filter per forward call).
+        Both serve forward calls only."""
         lib = L.load()
         d = L.ConvDesc()
         lib.mfft_conv2d_desc_default(ctypes.byref(d))
@@ -25,7 +28,9 @@ class Conv2d:
             d.pad_h, d.pad_w = (pad, pad) if isinstance(pad, int) else pad
         d.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         d.workspace_limit = workspace_limit
-        d.spec_fp16 = 1 if spec_fp16 else 0  # fp16 image spectra and intermediates (forward only)
+        if spec_fp16 and mid_fp16:
+            raise ValueError("spec_fp16 and mid_fp16 exclude each other")
+        d.spec_fp16 = L.CONV_SPEC_FP16 if spec_fp16 else L.CONV_SPEC_FP32_MID_FP16 if mid_fp16 else L.CONV_SPEC_FP32
         h_ = ctypes.c_void_p()
         L.check(lib.mfft_conv2d_create(ctypes.byref(h_), ctypes.byref(d)), "mfft_conv2d_create")
         self._h = h_
@@ -56,17 +61,26 @@ class Conv2d:
         L.check(L.load().mfft_conv2d_set_filters(self._h, ctypes.c_void_p(filt.data_ptr()), self._fmt(filt),
                                                   ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_set_filters")
 
-    def forward(self, x, out=None, stream=None):
+    def forward(self, x, out=None, stream=None, filter=None):
+        """All filters (out: [batch][filters][oh][ow]), or only filter index `filter`
+        (mfft_conv2d_forward_select; out: [batch][1][oh][ow])."""
         import torch
 
         d = self.desc
         assert x.is_cuda and x.is_contiguous() and x.numel() == d.batch * d.channels * d.h * d.w
         if out is None:
-            out = torch.empty(self.out_shape, dtype=x.dtype, device=x.device)
+            shape = self.out_shape if filter is None else (d.batch, 1) + self.out_shape[2:]
+            out = torch.empty(shape, dtype=x.dtype, device=x.device)
         s = stream if stream is not None else torch.cuda.current_stream(x.device)
-        L.check(L.load().mfft_conv2d_forward(self._h, ctypes.c_void_p(x.data_ptr()), self._fmt(x),
-                                              ctypes.c_void_p(out.data_ptr()), self._fmt(out),
-                                              ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_forward")
+        lib = L.load()
+        if filter is None:
+            L.check(lib.mfft_conv2d_forward(self._h, ctypes.c_void_p(x.data_ptr()), self._fmt(x),
+                                            ctypes.c_void_p(out.data_ptr()), self._fmt(out),
+                                            ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_forward")
+        else:
+            L.check(lib.mfft_conv2d_forward_select(self._h, ctypes.c_void_p(x.data_ptr()), self._fmt(x),
+                                                   ctypes.c_void_p(out.data_ptr()), self._fmt(out), int(filter),
+                                                   ctypes.c_void_p(s.cuda_stream)), "mfft_conv2d_forward_select")
         return out
 
     def forward_two(self, x0, x1, f0, f1, out0=None, out1=None, stream=None):

@@ -50,6 +50,10 @@ struct mfft_conv2d_s {
     int Ph = 0, Pw = 0, oh = 0, ow = 0, oy = 0, ox = 0;
     long long pairs = 0;
     int spec_fmt = MFFT_FMT_C32;  // image and product spectra X, Y (the filter spectra H stay fp32)
+    // fp32 spectra, fp16 pass boundaries (MFFT_CONV_SPEC_FP32_MID_FP16): the image transforms
+    // run out of place, so a transform whose input is not fused reads Y and writes X, and the
+    // inverse of a pointwise product in Y writes X
+    bool mid16 = false;
     mfft_plan fwd_img = nullptr, fwd_filt = nullptr, inv = nullptr;
     float2 *H = nullptr, *X = nullptr, *Y = nullptr;
     // backward (built on its first call): delta transform, inverses over (filter, channel)
@@ -158,10 +162,14 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     if (d->mode != MFFT_CONV_SAME && d->mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_INVALID, "mode");
     if (d->mode == MFFT_CONV_VALID_CORR && (d->fh > d->h || d->fw > d->w))
         return cfail(MFFT_ERR_INVALID, "filter larger than the image in 'valid' mode");
-    if (d->spec_fp16 != 0 && d->spec_fp16 != 1) return cfail(MFFT_ERR_INVALID, "spec_fp16");
+    if (d->spec_fp16 < MFFT_CONV_SPEC_FP32 || d->spec_fp16 > MFFT_CONV_SPEC_FP32_MID_FP16)
+        return cfail(MFFT_ERR_INVALID, "spec_fp16");
+    if (d->spec_fp16 == MFFT_CONV_SPEC_FP32_MID_FP16 && d->channels != 1)
+        return cfail(MFFT_ERR_UNSUPPORTED, "fp16 pass boundaries need a single-channel operator");
     mfft_conv2d op = new mfft_conv2d_s;
     op->d = *d;
-    op->spec_fmt = d->spec_fp16 ? MFFT_FMT_C16 : MFFT_FMT_C32;
+    op->spec_fmt = d->spec_fp16 == MFFT_CONV_SPEC_FP16 ? MFFT_FMT_C16 : MFFT_FMT_C32;
+    op->mid16 = d->spec_fp16 == MFFT_CONV_SPEC_FP32_MID_FP16;
     int need_h = d->h + d->fh - 1, need_w = d->w + d->fw - 1;
     if (d->mode == MFFT_CONV_VALID_CORR) {
         // circular wrap-around only touches outputs outside the 'valid' window
@@ -190,14 +198,20 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     mfft_desc_default(&fd);
     fd.precision = d->precision;
     fd.in_fmt = fd.out_fmt = op->spec_fmt;
-    if (op->spec_fmt == MFFT_FMT_C16) fd.mid_fmt = MFFT_FMT_C16;
+    if (op->spec_fmt == MFFT_FMT_C16 || op->mid16) fd.mid_fmt = MFFT_FMT_C16;
     fd.workspace_limit = d->workspace_limit;
     fd.scale = MFFT_SCALE_STAGE;  // image spectra carry 1/(Ph*Pw): the inverse is then unit
-    int rc = mfft_plan_create_2d(&op->fwd_img, op->Ph, op->Pw, op->pairs * d->channels, &fd);
+    // fp16 pass boundaries with fp32 ends: the intermediates the passes keep in the output
+    // buffer are fp16 as well (the image transforms then never run in place)
+    auto create = [&](mfft_plan *pl, int64_t b) {
+        return op->mid16 ? mfft::plan_create_2d_out_mid(pl, op->Ph, op->Pw, b, &fd)
+                         : mfft_plan_create_2d(pl, op->Ph, op->Pw, b, &fd);
+    };
+    int rc = create(&op->fwd_img, op->pairs * d->channels);
     if (!rc) {
         fd.direction = MFFT_INVERSE;
         fd.scale = MFFT_SCALE_NONE;
-        rc = mfft_plan_create_2d(&op->inv, op->Ph, op->Pw, op->pairs * d->filters, &fd);
+        rc = create(&op->inv, op->pairs * d->filters);
     }
     if (!rc) {
         // the resident filter spectra stay fp32 whatever the image spectra
@@ -251,6 +265,8 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
     if (!op || !in || !out) return cfail(MFFT_ERR_INVALID, "null argument");
     if (!op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
     if (fsel >= op->d.filters) return cfail(MFFT_ERR_INVALID, "filter %d of %d", fsel, op->d.filters);
+    if (op->mid16 && fsel < 0 && op->d.filters > 1)
+        return cfail(MFFT_ERR_UNSUPPORTED, "fp16 pass boundaries: one filter per call (mfft_conv2d_forward_select)");
     hipStream_t s = (hipStream_t)stream;
     OpCall call(op, s);
     if (call.err != hipSuccess) return cfail(MFFT_ERR_HIP, "stream order: %s", hipGetErrorString(call.err));
@@ -267,17 +283,20 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
         f.real_w = d.w;
         rc = mfft::exec_fused(op->fwd_img, op->X, op->X, op->pairs * d.channels, s, f);
     } else {
+        // Y holds at least as many spectra as X (one channel whenever mid16)
+        void *packed = op->mid16 ? (void *)op->Y : (void *)op->X;
         mfft::trace_launch(s, "conv_pack", (long long)d.batch * d.channels * d.h * d.w * (in_fmt == MFFT_REAL_F64 ? 8 : 4),
                            op->pairs * d.channels * (long long)op->Ph * op->Pw * spec_eb(op));
-        if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, op->X, op->spec_fmt, d.batch, d.channels, d.h, d.w,
+        if (mfft::conv_pack_pairs(in, in_fmt == MFFT_REAL_F64, packed, op->spec_fmt, d.batch, d.channels, d.h, d.w,
                                   op->Ph, op->Pw, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack");
-        rc = mfft_exec(op->fwd_img, op->X, op->X, s);
+        rc = mfft_exec(op->fwd_img, packed, op->X, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
     const long long P = (long long)op->Ph * op->Pw;
     const int nf = fsel >= 0 ? 1 : d.filters;
     const float2 *H = fsel >= 0 ? op->H + (long long)fsel * d.channels * P : op->H;
+    float2 *res = op->Y;  // the inverse's output
     if (d.channels == 1 && nf == 1 && mfft::plan_first_pass_fusable(op->inv)) {
         // one filter, one channel: the inverse's first pass multiplies by the spectrum
         mfft::ExecFuse f;
@@ -289,12 +308,13 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
                            op->pairs * nf * P * spec_eb(op));
         if (mfft::conv_pointwise(op->X, H, op->Y, op->spec_fmt, op->pairs, d.channels, nf, P, s) != hipSuccess)
             return cfail(MFFT_ERR_HIP, "pointwise");
-        rc = mfft_exec_ex(op->inv, op->Y, op->Y, op->pairs * nf, nullptr, s);
+        if (op->mid16) res = op->X;  // out of place; X holds pairs x 1 spectra and nf == 1 here
+        rc = mfft_exec_ex(op->inv, op->Y, res, op->pairs * nf, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
     mfft::trace_launch(s, "conv_crop", op->pairs * nf * (long long)op->oh * op->ow * spec_eb(op),
                        (long long)d.batch * nf * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
-    if (mfft::conv_crop(op->Y, op->spec_fmt, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
+    if (mfft::conv_crop(res, op->spec_fmt, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
                         op->Pw, op->oy, op->ox, 1.0f, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
@@ -337,22 +357,24 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
         f.real_w = d.w;
         rc = mfft::exec_fused(op->fwd_img, op->X, op->X, 1, s, f);
     } else {
+        void *packed = op->mid16 ? (void *)op->Y : (void *)op->X;
         mfft::trace_launch(s, "conv_pack_two", 2LL * d.h * d.w * (in_fmt == MFFT_REAL_F64 ? 8 : 4), P * spec_eb(op));
-        if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, op->X, op->spec_fmt, d.h, d.w, op->Ph, op->Pw, s) !=
+        if (mfft::conv_pack_two(in0, in1, in_fmt == MFFT_REAL_F64, packed, op->spec_fmt, d.h, d.w, op->Ph, op->Pw, s) !=
             hipSuccess)
             return cfail(MFFT_ERR_HIP, "pack");
-        rc = mfft_exec_ex(op->fwd_img, op->X, op->X, 1, nullptr, s);
+        rc = mfft_exec_ex(op->fwd_img, packed, op->X, 1, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
     mfft::trace_launch(s, "conv_pointwise_two", P * spec_eb(op) + 2 * P * 8, P * spec_eb(op));
     if (mfft::conv_pointwise_two(op->X, op->H + filter0 * P, op->H + filter1 * P, op->Y, op->spec_fmt, op->Ph, op->Pw,
                                  s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "pointwise");
-    rc = mfft_exec_ex(op->inv, op->Y, op->Y, 1, nullptr, s);
+    float2 *res = op->mid16 ? op->X : op->Y;  // mid16: the inverse runs out of place
+    rc = mfft_exec_ex(op->inv, op->Y, res, 1, nullptr, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
     mfft::trace_launch(s, "conv_crop_two", (long long)op->oh * op->ow * spec_eb(op),
                        2LL * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
-    if (mfft::conv_crop_two(op->Y, op->spec_fmt, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy,
+    if (mfft::conv_crop_two(res, op->spec_fmt, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy,
                             op->ox, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
@@ -397,7 +419,8 @@ int mfft_conv2d_backward(mfft_conv2d op, const void *in, const void *delta, int 
 {
     if (!op || !delta || (!grad_in && !grad_w) || (grad_w && !in)) return cfail(MFFT_ERR_INVALID, "null argument");
     if (op->d.mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_UNSUPPORTED, "backward of the 'valid' correlation mode only");
-    if (op->spec_fmt != MFFT_FMT_C32) return cfail(MFFT_ERR_UNSUPPORTED, "backward needs fp32 spectra (spec_fp16 = 0)");
+    if (op->d.spec_fp16 != MFFT_CONV_SPEC_FP32)
+        return cfail(MFFT_ERR_UNSUPPORTED, "backward needs fp32 spectra and intermediates (spec_fp16 = 0)");
     if (grad_in && !op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
     hipStream_t s = (hipStream_t)stream;
     OpCall call(op, s);

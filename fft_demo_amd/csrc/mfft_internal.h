@@ -19,6 +19,12 @@ struct ExecFuse {
     int real_f64 = 0, real_b = 0, real_c = 0, real_h = 0, real_w = 0;
 };
 
+// mfft_plan_create_2d whose passes keep the intermediates they store in the output buffer in
+// the desc's mid_fmt (only the last pass writes out_fmt there): 16-bit pass boundaries for a
+// plan with 32-bit input and output. Its execs need out != in unless the first pass reads a
+// fused real source instead of the input.
+int plan_create_2d_out_mid(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d);
+
 // Whether the plan's first pass can take an ExecFuse.
 bool plan_first_pass_fusable(mfft_plan p);
 // mfft_exec_ex with the fused first pass (no caller workspace); MFFT_ERR_UNSUPPORTED when

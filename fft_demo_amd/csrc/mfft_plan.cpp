@@ -241,7 +241,26 @@ struct mfft_plan_s {
     // small_below transforms, which would leave most of the device idle on the on-chip pass.
     mfft_plan_s *small = nullptr;
     long long small_below = 0;
+    // Intermediates that a pass stores in the output buffer are kept in mid_fmt (the last
+    // pass writes out_fmt there): with 16-bit intermediates and 32-bit output, every pass
+    // boundary then moves 16-bit words. Execs must not alias a pass's source and destination
+    // across a format change (checked at exec).
+    bool out_mid = false;
 };
+
+// Element formats a pass reads and writes.
+static int src_fmt(const mfft_plan_s *p, const PassSpec &ps)
+{
+    if (ps.src == BUF_IN) return p->d.in_fmt;
+    if (ps.src == BUF_OUT) return p->out_mid ? p->mid_fmt : p->d.out_fmt;
+    return p->mid_fmt;
+}
+static int dst_fmt(const mfft_plan_s *p, const PassSpec &ps)
+{
+    if (ps.dst == BUF_OUT) return p->out_mid && &ps != &p->passes.back() ? p->mid_fmt : p->d.out_fmt;
+    if (ps.dst == BUF_IN) return p->d.in_fmt;
+    return p->mid_fmt;
+}
 
 // Transforms (lines, or images of 2048-row 2D plans) an on-chip pass needs to occupy the
 // device enough to beat the multi-pass plan: a workgroup per 65536-point line, 2/3 of the
@@ -589,8 +608,10 @@ static int finish_passes(mfft_plan_s *p)
         }
         bool strided = ps.ps_in != 1;
         bool pow2 = is_pow2(ps.R) && ps.R >= 16;
-        const int in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
-        const int out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+        const int in_fmt = src_fmt(p, ps);
+        const int out_fmt = dst_fmt(p, ps);
+        if (p->out_mid && ps.src == ps.dst && in_fmt != out_fmt)
+            return fail(MFFT_ERR_UNSUPPORTED, "in-place pass changes the element format");
         // digit-reversed lines on the fast kernels: the list starts with 16 and the rest
         // reverses digit by digit (every reference list does)
         ps.rev_fast = rev && fast_rev_ok(p->d.precision, in_fmt) && fast_rev_spec(ps.R, ps.rev_list, ps.rev_q);
@@ -805,7 +826,10 @@ static int finalize_plan(mfft_plan_s *p)
     p->chunk = p->batch;
     if (need_work) {
         long long c = std::max(1LL, limit / std::max(1LL, per));
-        p->chunk = std::min(p->batch, c);
+        // as many chunks as the limit needs, of equal size (8 transforms under a limit of 7:
+        // 4 + 4, not 7 + 1, whose one-transform launch would leave most of the device idle)
+        const long long nch = (p->batch + c - 1) / c;
+        p->chunk = std::min(p->batch, (p->batch + nch - 1) / nch);
         p->work_bytes = p->chunk * per;
         if (!p->d.external_workspace && alloc_dev(p, &p->work, (size_t)p->work_bytes)) return MFFT_ERR_NOMEM;
     }
@@ -893,8 +917,8 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         }
         a.in = buf_ptr(ps.src, in, out, work);
         a.out = buf_ptr(ps.dst, in, out, work);
-        a.in_fmt = ps.src == BUF_IN ? p->d.in_fmt : ps.src == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
-        a.out_fmt = ps.dst == BUF_OUT ? p->d.out_fmt : ps.dst == BUF_IN ? p->d.in_fmt : p->mid_fmt;
+        a.in_fmt = src_fmt(p, ps);
+        a.out_fmt = dst_fmt(p, ps);
         a.R = ps.R;
         a.nstages = (int)ps.radix.size();
         for (int i = 0; i < a.nstages; ++i) {
@@ -1134,7 +1158,8 @@ int mfft_plan_create_1d(mfft_plan *plan, int n, int64_t batch, const mfft_desc *
     return MFFT_OK;
 }
 
-static int create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in, bool allow_onchip)
+static int create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in, bool allow_onchip,
+                     bool out_mid = false)
 {
     if (!plan || !d_in) return fail(MFFT_ERR_INVALID, "null argument");
     *plan = nullptr;
@@ -1196,6 +1221,7 @@ static int create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_
                            rev_cols);
         }
     }
+    p->out_mid = out_mid;
     if (!rc) rc = finalize_plan(p);
     if (rc) {
         std::string msg = g_err;
@@ -1208,13 +1234,13 @@ static int create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_
     return MFFT_OK;
 }
 
-int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in)
+static int create_2d_all(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in, bool out_mid)
 {
-    int rc = create_2d(plan, nx, ny, batch, d_in, true);
+    int rc = create_2d(plan, nx, ny, batch, d_in, true, out_mid);
     if (rc || !has_onchip(*plan)) return rc;
     mfft_plan p = *plan, q = nullptr;
     const long long below = onchip_cols_min_images(ny);
-    rc = create_2d(&q, nx, ny, below - 1, d_in, false);
+    rc = create_2d(&q, nx, ny, below - 1, d_in, false, out_mid);
     if (rc) {
         std::string msg = g_err;
         mfft_plan_destroy(p);
@@ -1225,6 +1251,11 @@ int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mf
     p->small = q;
     p->small_below = below;
     return MFFT_OK;
+}
+
+int mfft_plan_create_2d(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d_in)
+{
+    return create_2d_all(plan, nx, ny, batch, d_in, false);
 }
 
 int mfft_plan_destroy(mfft_plan plan)
@@ -1262,11 +1293,8 @@ int mfft_plan_info(mfft_plan p, mfft_info *info)
         long long c0 = ps.lines[ps.l0].batch ? p->chunk : ps.lines[ps.l0].count;
         pi.tiles = (lines / c0) * ((c0 + ps.T - 1) / ps.T);
         pi.generic = ps.generic;
-        auto fmt_of = [&](int b) {
-            return b == BUF_IN ? p->d.in_fmt : b == BUF_OUT ? p->d.out_fmt : p->mid_fmt;
-        };
-        pi.in_elem_bytes = fmt_bytes(fmt_of(ps.src));
-        pi.out_elem_bytes = fmt_bytes(fmt_of(ps.dst));
+        pi.in_elem_bytes = fmt_bytes(src_fmt(p, ps));
+        pi.out_elem_bytes = fmt_bytes(dst_fmt(p, ps));
         pi.points = lines * ps.R;
         pi.kernel = ps.small ? MFFT_KERNEL_STREAM
                     : ps.fast_kind == KIND_ROWS ? MFFT_KERNEL_ROWS
@@ -1291,6 +1319,14 @@ static int exec_all(mfft_plan p, const void *in, void *out, void *stream, hipEve
     if (p->small && batch >= 0 && batch < p->small_below && only < 0 && !ev &&
         (!fz || (p->small->passes.front().small && p->small->passes.front().src == BUF_IN)))
         return exec_all(p->small, in, out, stream, nullptr, batch, work, fz, -1, nullptr, nullptr);
+    if (p->out_mid && in == out)
+        for (const PassSpec &ps : p->passes) {
+            // a pass that reads the input (not a fused real source) and writes the output in
+            // another format would overwrite lines it has not read yet
+            const bool reads = !(fz && fz->real_src && &ps == &p->passes.front());
+            if (reads && ps.src == BUF_IN && ps.dst == BUF_OUT && src_fmt(p, ps) != dst_fmt(p, ps))
+                return fail(MFFT_ERR_INVALID, "plan keeps intermediates in the output buffer: in and out must differ");
+        }
     std::unique_lock<std::mutex> lk(p->exec_mu, std::defer_lock);
     if (!work) lk.lock();  // the plan's own workspace is shared by its callers
     hipStream_t s = (hipStream_t)stream;
@@ -1694,6 +1730,11 @@ void StreamOrder::destroy()
 {
     if (ev) (void)hipEventDestroy(ev);
     ev = nullptr;
+}
+
+int plan_create_2d_out_mid(mfft_plan *plan, int nx, int ny, int64_t batch, const mfft_desc *d)
+{
+    return create_2d_all(plan, nx, ny, batch, d, true);
 }
 
 bool plan_first_pass_fusable(mfft_plan p)

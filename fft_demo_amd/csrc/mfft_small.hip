@@ -213,13 +213,18 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
         dft_odd<R>(v, a.stage_tw);
     else
         dft<R>(v, a.sign, a.w16);
+    // four-step twiddle W_N^(mult k): W_N^mult from the two-level table (FP64-generated), its
+    // powers by repeated products (fewer than R <= 17 fp32 roundings; two table reads per
+    // line instead of two per point)
     const unsigned mult = (unsigned)(a.tw_level ? l1 : l0);
+    float2 w1 = make_float2(1.f, 0.f), wk = w1;
+    if (a.tw_N) w1 = cm(a.tw_hi[mult >> a.tw_bits], a.tw_lo[mult & ((1u << a.tw_bits) - 1u)]);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         float2 y = make_float2(v[k].x * a.last_mul, v[k].y * a.last_mul);
         if (a.tw_N && k) {
-            const unsigned e = mult * (unsigned)k;
-            y = cm(y, cm(a.tw_hi[e >> a.tw_bits], a.tw_lo[e & ((1u << a.tw_bits) - 1u)]));
+            wk = k == 1 ? w1 : cm(wk, w1);
+            y = cm(y, wk);
         }
         st<OUTF>(a.out, ob + (long long)k * a.out_ps, y);
     }

@@ -215,6 +215,7 @@ int mfft_stage_convert(const void *src, int src_fmt, void *dst, int dst_fmt, int
  * ------------------------------------------------------------------------------------ */
 enum { MFFT_CONV_SAME = 0, MFFT_CONV_VALID_CORR = 1 };
 enum { MFFT_REAL_F32 = 0, MFFT_REAL_F64 = 1 };
+enum { MFFT_CONV_SPEC_FP32 = 0, MFFT_CONV_SPEC_FP16 = 1, MFFT_CONV_SPEC_FP32_MID_FP16 = 2 };
 
 typedef struct mfft_conv2d_desc {
     int batch, channels, filters;
@@ -222,10 +223,14 @@ typedef struct mfft_conv2d_desc {
     int mode;           /* MFFT_CONV_* */
     int pad_h, pad_w;   /* transform size; 0 = smallest supported size >= h + fh - 1 */
     int precision;      /* MFFT_PREC_* of the transforms (default split fp16) */
-    int spec_fp16;      /* 0 (default): image spectra and the transforms' intermediates in fp32;
-                           1: in fp16 (half the bytes of every pass; each pass boundary rounds
-                           to fp16, rel-L2 of the outputs about 1e-4..5e-4). Forward calls
-                           only: mfft_conv2d_backward needs fp32 spectra. */
+    int spec_fp16;      /* MFFT_CONV_SPEC_*. FP32 (0, default): image spectra and the
+                           transforms' intermediates in fp32. FP16 (1): both in fp16 (half the
+                           bytes of every pass; each pass boundary rounds to fp16, rel-L2 of
+                           the outputs about 1e-4..5e-4). FP32_MID_FP16 (2): fp32 image spectra
+                           and transform outputs, fp16 words at the pass boundaries inside each
+                           transform; single-channel operators, one filter per call (filters
+                           = 1 or mfft_conv2d_forward_select). FP16 and FP32_MID_FP16 serve
+                           forward calls only: mfft_conv2d_backward needs FP32. */
     int64_t workspace_limit;
 } mfft_conv2d_desc;
 

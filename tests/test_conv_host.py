@@ -3,6 +3,7 @@ and the message every C entry point leaves in mfft_last_error()."""
 import ctypes
 
 ERR_INVALID = -1  # MFFT_ERR_INVALID in include/mfft.h
+ERR_UNSUPPORTED = -2  # MFFT_ERR_UNSUPPORTED
 
 
 def test_conv_create_errors_reach_last_error():
@@ -17,9 +18,15 @@ def test_conv_create_errors_reach_last_error():
     assert lib.mfft_conv2d_create(ctypes.byref(h), ctypes.byref(d)) == ERR_INVALID
     assert b"bad sizes" in lib.mfft_last_error()
     d.h, d.w, d.fh, d.fw = 32, 32, 3, 3
-    d.spec_fp16 = 2
+    d.spec_fp16 = 3
     assert lib.mfft_conv2d_create(ctypes.byref(h), ctypes.byref(d)) == ERR_INVALID
     assert b"spec_fp16" in lib.mfft_last_error()
+    # fp32 spectra with fp16 pass boundaries: single-channel operators only
+    d.spec_fp16 = _lib.CONV_SPEC_FP32_MID_FP16
+    d.channels = 2
+    assert lib.mfft_conv2d_create(ctypes.byref(h), ctypes.byref(d)) == ERR_UNSUPPORTED
+    assert b"single-channel" in lib.mfft_last_error()
+    d.channels = 1
     d.spec_fp16 = 0
     d.mode = 7
     assert lib.mfft_conv2d_create(ctypes.byref(h), ctypes.byref(d)) == ERR_INVALID

@@ -161,10 +161,12 @@ def _gauss31():
 
 
 @pytest.mark.gpu
-def test_c4_conv_same_fp16_spectra_pad2176(dev):
-    """The bench's conv operator: 16 images of 2048 x 2048 (U[0,1], seed 42), one 31 x 31
-    Gaussian filter, pad 2176, split fp16 transforms with fp16 image spectra. Row bands at
-    the top, middle and bottom (all columns: both side borders) of images 0 and 15."""
+@pytest.mark.parametrize("mode", ["spec_fp16", "mid_fp16"])
+def test_c4_conv_same_fp16_spectra_pad2176(dev, mode):
+    """The bench's conv operators: 16 images of 2048 x 2048 (U[0,1], seed 42), one 31 x 31
+    Gaussian filter, pad 2176, split fp16 transforms with fp16 image spectra (spec_fp16) or
+    fp32 image spectra and fp16 pass boundaries (mid_fp16). Row bands at the top, middle and
+    bottom (all columns: both side borders) of images 0 and 15."""
     import torch
 
     from fft_demo_amd.conv import Conv2d
@@ -172,7 +174,7 @@ def test_c4_conv_same_fp16_spectra_pad2176(dev):
     B, H, W, K = 16, 2048, 2048, 31
     img = oracle.uniform(B * H * W, 42, 0.0, 1.0).astype(np.float32).reshape(B, 1, H, W)
     g = _gauss31()
-    op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision="split_fp16", spec_fp16=True)
+    op = Conv2d(B, 1, 1, H, W, K, K, mode="same", precision="split_fp16", **{mode: True})
     assert tuple(op.pad) == (2176, 2176)
     op.set_filters(torch.from_numpy(g.astype(np.float32)).to(dev).reshape(1, 1, K, K).contiguous())
     out = op.forward(torch.from_numpy(img).to(dev))
@@ -182,7 +184,7 @@ def test_c4_conv_same_fp16_spectra_pad2176(dev):
             ref = oracle.conv2d_same(img[b, 0].astype(np.float64), g, rows=(r0, r0 + 16))
             got = out[b, 0, r0:r0 + 16].double().cpu().numpy()
             l2, mx = oracle.err_l2max(got, ref)
-            print(f"conv 2048^2 spec_fp16 image {b} rows {r0}..{r0 + 15}: rel_l2={l2:.2e} rel_max={mx:.2e}")
+            print(f"conv 2048^2 {mode} image {b} rows {r0}..{r0 + 15}: rel_l2={l2:.2e} rel_max={mx:.2e}")
             assert l2 <= 1e-3 and mx <= 1e-2
     op.close()
 

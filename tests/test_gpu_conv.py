@@ -204,6 +204,62 @@ def test_conv_same_fp16_spectra(dev, B, C, F, h, w, fh, fw):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,h,w,fh,fw,pad", [(3, 120, 136, 9, 7, None), (2, 100, 90, 5, 5, (128, 128))])
+def test_conv_same_fp32_spectra_fp16_boundaries(dev, B, h, w, fh, fw, pad):
+    """fp32 image spectra with fp16 pass boundaries inside each transform (mid_fp16), three
+    filters, each selected per call; padded 128 x 144 (odd part 9: image load and filter
+    product fused into the first passes) and 128 x 128 (no fused pass: the packed images and
+    the pointwise product go through the transforms out of place). Tolerance as for
+    spec_fp16: rel-L2 <= 1e-3 and rel-max <= 1e-2 of each output map. A call for all filters
+    at once is refused (one filter per call)."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    F = 3
+    img = oracle.uniform(B * h * w, 91, 0.0, 1.0).reshape(B, 1, h, w)
+    filt = oracle.uniform(F * fh * fw, 92, -1.0, 1.0).reshape(F, 1, fh, fw)
+    op = Conv2d(B, 1, F, h, w, fh, fw, mode="same", pad=pad, mid_fp16=True)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=torch.float32).contiguous())
+    x = torch.from_numpy(img).to(device=dev, dtype=torch.float32).contiguous()
+    for f in (2, 0):
+        got = op.forward(x, filter=f).double().cpu().numpy()
+        for b in range(B):
+            l2, mx = oracle.err_l2max(got[b, 0], oracle.conv2d_same(img[b, 0], filt[f, 0]))
+            print(f"mid_fp16 pad={op.pad} b={b} f={f} rel_l2={l2:.2e} rel_max={mx:.2e}")
+            assert l2 <= 1e-3 and mx <= 1e-2, (b, f, l2, mx)
+    with pytest.raises(RuntimeError, match="one filter per call"):
+        op.forward(x)
+    op.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pad", [None, (256, 256)])
+def test_conv_forward_two_fp32_spectra_fp16_boundaries(dev, pad):
+    """forward_two on a mid_fp16 operator (fused and unfused first passes) against the
+    oracle, tolerance as above; its backward pass is refused."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    h, w, k, F = 200, 150, 7, 3
+    filt = oracle.uniform(F * k * k, 93, -1.0, 1.0).reshape(F, 1, k, k)
+    imgs = [oracle.uniform(h * w, 94 + i, 0.0, 1.0).reshape(h, w) for i in range(2)]
+    op = Conv2d(1, 1, F, h, w, k, k, mode="valid_corr", pad=pad, mid_fp16=True)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=torch.float32).contiguous())
+    xs = [torch.from_numpy(im).to(device=dev, dtype=torch.float32).contiguous() for im in imgs]
+    o0, o1 = op.forward_two(xs[0], xs[1], 1, 2)
+    torch.cuda.synchronize()
+    for im, f, o in [(imgs[0], 1, o0), (imgs[1], 2, o1)]:
+        l2, mx = oracle.err_l2max(o.double().cpu().numpy(), oracle.corr2d_valid(im, filt[f, 0]))
+        print(f"forward_two mid_fp16 pad={op.pad} f={f} rel_l2={l2:.2e} rel_max={mx:.2e}")
+        assert l2 <= 1e-3 and mx <= 1e-2
+    with pytest.raises(RuntimeError, match="fp32 spectra"):
+        op.backward(torch.zeros(1, F, h - k + 1, w - k + 1, device=dev), xs[0].reshape(1, 1, h, w))
+    op.close()
+
+
+@pytest.mark.gpu
 def test_conv_forward_two_fp16_spectra_and_backward_refused(dev):
     """forward_two with fp16 spectra against the oracle (tolerance as for spec_fp16 above);
     the backward pass of such an operator is refused (it needs fp32 spectra)."""
