@@ -374,6 +374,22 @@ def _cpu_model():
     return "unknown"
 
 
+def same_size_copy(nbytes, steps, stream):
+    """A device-to-device copy of nbytes (read nbytes, write nbytes) timed like the FFT
+    execs (time_execs: back-to-back between HIP events, or one hipGraph of `steps` copies
+    when that is faster): the time a launch moving the same bytes takes at this size,
+    launch gaps included. Returns (us per copy, GB/s of read + write bytes, timing)."""
+    import torch
+
+    a = torch.ones(nbytes // 4, device="cuda")
+    b = torch.empty_like(a)
+    settle(lambda st: b.copy_(a), stream)
+    total, how = time_execs(lambda st: b.copy_(a), steps, stream)
+    del a, b
+    us = total / steps * 1e3
+    return round(us, 3), round(2 * nbytes / (us * 1e-6) / 1e9, 1), how
+
+
 def copy_bandwidth(mib=1024, reps=10):
     """Practical HBM ceiling on this device: a device-to-device copy of `mib` MiB, read +
     write bytes per second (GB/s)."""
@@ -575,7 +591,18 @@ def compact_fft(cfg, steps, warmup, dev, stream, with_cpu=True, with_ref=False):
         out["cpu_baseline"] = cpu_baseline(shape, batch, xh)
     if with_ref and len(shape) == 1:
         out["reference_api"] = reference_api(shape[0], batch, xh, steps, warmup, dev)
+    if head["ms_per_exec"] < 0.05:
+        add_same_size_copy(out["roofline"], steps, stream)
     return out
+
+
+def add_same_size_copy(rl, steps, stream):
+    """A launch this short is bound by its fixed costs: a copy of the same bytes, timed the
+    same way, is the practical ceiling at this size."""
+    us, gbs, how = same_size_copy(rl["algorithmic_bytes_per_launch"] // 2, steps, stream)
+    rl["same_size_copy"] = {"us": us, "gbs": gbs, "timing": how,
+                            "bytes": f"{rl['algorithmic_bytes_per_launch'] // 2} read + as many written"}
+    rl["frac_of_same_size_copy"] = round(rl["achieved"] / gbs, 4)
 
 
 # ------------------------------------------------------------------------- MFMA use
@@ -956,6 +983,8 @@ def headline_line(cfg, args, dev, stream):
     roofline = roofline_of(head)
     roofline["copy_gbs"] = copy_bandwidth()
     roofline["frac_of_copy"] = round(roofline["achieved"] / roofline["copy_gbs"], 4)
+    if head["ms_per_exec"] < 0.05:
+        add_same_size_copy(roofline, args.steps, stream)
     n = math.prod(shape)
     comps_out = {}
     for k, v in comps.items():

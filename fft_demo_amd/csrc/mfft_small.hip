@@ -141,7 +141,17 @@ __global__ void __launch_bounds__(256) small_cols(TileArgs a)
     // line index -> (l0, l1, l2); 32-bit divisions whenever the line count allows
     int l0, l1;
     long long l2;
-    if (nl <= 0xffffffffLL) {
+    if (a.in_mul && nl <= 0xffffffffLL) {
+        // conv filter product: l2 (the transforms of the launch) varies fastest, so the lanes
+        // that multiply by one word of the filter spectrum sit in one wave and the word is
+        // fetched once per wave rather than once per transform
+        const unsigned u = (unsigned)id, c2 = (unsigned)a.c2, c0 = (unsigned)a.c0;
+        const unsigned r = u / c2;
+        l2 = u - r * c2;
+        const unsigned q = r / c0;
+        l0 = (int)(r - q * c0);
+        l1 = (int)q;
+    } else if (nl <= 0xffffffffLL) {
         const unsigned u = (unsigned)id, c0 = (unsigned)a.c0, c1 = (unsigned)a.c1;
         const unsigned r = u / c0;
         l0 = (int)(u - r * c0);
