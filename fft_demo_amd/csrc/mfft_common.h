@@ -28,6 +28,13 @@ constexpr int kThreads = kWaves * 64;
 constexpr int kMaxTile = 4096;            // complex points per workgroup tile
 constexpr int kMaxPairsPerWave = kMaxTile / 256 / kWaves;  // MFMA column groups per wave/stage
 constexpr int kMaxValuPerThread = 32;
+// Cache policy of the on-chip kernels' output stores: nt (non-temporal, bit 1 of the buffer
+// instructions' policy operand on gfx950). Their outputs are a plan's final result, which no
+// later pass of the plan reads: 65536 x 256 ran 35.1 -> 33.8 us per launch in isolation, 36.2
+// -> 36.0 us per exec in the bench. Stores of intermediates keep the default policy: with nt on
+// the multi-pass kernels the next pass read its input from HBM instead of the caches (1024^2 x
+// 64 2D 9% and the 2048^2 conv 5% slower), and nt loads made the on-chip kernel 7% slower.
+constexpr int kStoreStream = 2;
 constexpr int kFastTw = 256;              // four-step table entries staged in LDS (each half)
 constexpr int kFastTwCols = 1024;         // the same for the strided kind (the four-step's first
                                           // pass, where the twiddles are applied; N <= 2^20)

@@ -288,7 +288,8 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
                 for (int i = 0; i < 4; ++i)
                     __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(make_float2(ar[i] * fac, ai[i] * fac)), dst,
                                                           (int)(4 * st_off),
-                                                          4 * (ROWS * r + CH * hb + SIDE * (f + 64 * i)), 0);
+                                                          4 * (ROWS * r + CH * hb + SIDE * (f + 64 * i)),
+                                                          kStoreStream);
             }
         }
     }
@@ -493,7 +494,8 @@ __global__ void __launch_bounds__(NT) fft_onchip_cols(OnchipColsArgs a)
                 for (int ka = 0; ka < 8; ++ka) {
                     const int k = khi + 16 * (tkb + 16 * ka);
                     const float2 y = make_float2(v[ka].x * a.tail_mul, v[ka].y * a.tail_mul);
-                    __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(y), rd, 4 * tcol, 4 * k * ps_out, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(y), rd, 4 * tcol, 4 * k * ps_out,
+                                                          kStoreStream);
                 }
             }
         }
