@@ -28,12 +28,14 @@ constexpr int kThreads = kWaves * 64;
 constexpr int kMaxTile = 4096;            // complex points per workgroup tile
 constexpr int kMaxPairsPerWave = kMaxTile / 256 / kWaves;  // MFMA column groups per wave/stage
 constexpr int kMaxValuPerThread = 32;
-// Cache policy of the on-chip kernels' output stores: nt (non-temporal, bit 1 of the buffer
-// instructions' policy operand on gfx950). Their outputs are a plan's final result, which no
-// later pass of the plan reads: 65536 x 256 ran 35.1 -> 33.8 us per launch in isolation, 36.2
-// -> 36.0 us per exec in the bench. Stores of intermediates keep the default policy: with nt on
-// the multi-pass kernels the next pass read its input from HBM instead of the caches (1024^2 x
-// 64 2D 9% and the 2048^2 conv 5% slower), and nt loads made the on-chip kernel 7% slower.
+// Cache policy of the 2048-point on-chip line kernels' output stores: nt (non-temporal, bit
+// 1 of the buffer instructions' policy operand on gfx950). Their outputs are a plan's final
+// result, which no later pass of the plan reads, and each store instruction writes whole
+// 128-byte row segments: 2048^2 x 16 columns 71.7 -> 67.2 us per launch, 2^20 x 64 rows 69.3
+// -> 65.1 us, HBM traffic unchanged. Kept off elsewhere: the 65536-point kernel's 64-byte store
+// segments then wrote 14% more bytes for no measurable gain, multi-pass kernels' intermediates
+// are read by the next pass from the caches (with nt, 1024^2 x 64 2D ran 9% and the 2048^2
+// conv 5% slower), and nt loads made the 65536-point kernel 7% slower.
 constexpr int kStoreStream = 2;
 constexpr int kFastTw = 256;              // four-step table entries staged in LDS (each half)
 constexpr int kFastTwCols = 1024;         // the same for the strided kind (the four-step's first

@@ -288,8 +288,7 @@ __global__ void __launch_bounds__(NT) fft_onchip(OnchipArgs a)
                 for (int i = 0; i < 4; ++i)
                     __builtin_amdgcn_raw_buffer_store_b32(Elem<FMT>::pack(make_float2(ar[i] * fac, ai[i] * fac)), dst,
                                                           (int)(4 * st_off),
-                                                          4 * (ROWS * r + CH * hb + SIDE * (f + 64 * i)),
-                                                          kStoreStream);
+                                                          4 * (ROWS * r + CH * hb + SIDE * (f + 64 * i)), 0);
             }
         }
     }
