@@ -1,0 +1,110 @@
+"""Limits and argument errors of the plan/exec entry points on the GPU.
+
+The longest 1D lines of the reference's size list, 2^24 and 2^27 points (tcfftCreate's
+largest cases, tcfft_half.cu:1296-1326), are too long for the FP64 oracle in a test, so they
+are checked through properties that hold at any size: a forward then inverse round trip
+returns the input, Parseval's identity, and an impulse's closed-form transform on sampled
+frequencies (split fp16 operands, fp16 input and intermediates, fp32 output: at 2^27 points the
+per-stage 1/16 scaling leaves outputs near 5e-5, where fp16 words would be subnormal). The
+impulse has amplitude 2^14: the scaling divides a unit impulse by the length done so far, so
+after two 256-point passes its 16-bit intermediate would be 2^-16, below fp16's normal range
+(6.1e-5), where the words keep a fixed absolute precision (2^-24) instead of 11 bits.
+
+Invalid sizes, batches and buffers are refused with an error code and a message (the
+reference's switch keeps stale state for a size it has no case for, tcfft_half.cu:1250-1330).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+ERR_INVALID, ERR_UNSUPPORTED = -1, -2  # MFFT_ERR_* in include/mfft.h
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1 << 24, 1 << 27])
+def test_longest_lines_round_trip_parseval_impulse(dev, n):
+    import torch
+
+    import fft_demo_amd as F
+
+    fwd = F.Plan((n,), 1, precision="split_fp16", in_fmt="c16", mid_fmt="c16", out_fmt="c32")
+    inv = F.Plan((n,), 1, precision="split_fp16", in_fmt="c32", mid_fmt="c16", out_fmt="c32", scale=0, direction=1)
+    s = fwd.out_scale
+    assert abs(s * n - 1.0) < 1e-12  # per-stage 1/radix: forward = DFT / n
+    print(f"n = 2^{n.bit_length() - 1}: passes {[(p['kernel'], p['R']) for p in fwd.info['passes']]}")
+
+    g = torch.Generator(device=dev).manual_seed(1234)
+    x = (torch.rand((1, n, 2), generator=g, device=dev) * 2 - 1).half()
+    X = fwd.exec(x)
+    back = inv.exec(X)
+    torch.cuda.synchronize()
+    xf = x.float()
+    rt = (torch.linalg.vector_norm(back - xf) / torch.linalg.vector_norm(xf)).item()
+    # Parseval with the 1/n scaling: ||X||^2 = ||x||^2 / n
+    par = (torch.sum(X.double() ** 2) * n / torch.sum(xf.double() ** 2)).item()
+    print(f"round trip rel-L2 {rt:.3e}; Parseval ratio {par:.6f}")
+    assert rt <= 2e-3
+    assert abs(par - 1.0) <= 2e-3
+
+    # impulse of amplitude A at n0: X[k] = A exp(-2 pi i n0 k / n) / n, on 65536 sampled
+    # frequencies
+    n0, amp = 12345 % n, 2.0 ** 14
+    imp = torch.zeros((1, n, 2), dtype=torch.float16, device=dev)
+    imp[0, n0, 0] = amp
+    Y = fwd.exec(imp)
+    torch.cuda.synchronize()
+    k = np.random.default_rng(7).choice(n, 65536, replace=False)
+    k[:3] = [0, 1, n - 1]
+    kt = torch.from_numpy(k).to(dev)
+    got = Y[0, kt].double().cpu().numpy()
+    got = (got[:, 0] + 1j * got[:, 1]) * (n / amp)
+    ref = np.exp(-2j * np.pi * ((n0 * k) % n) / n)
+    err = np.abs(got - ref).max()
+    print(f"impulse at {n0}: max |n X[k] / A - W^(n0 k)| over 65536 frequencies = {err:.3e}")
+    assert err <= 5e-3
+    for p in (fwd, inv):
+        p.close()
+
+
+@pytest.mark.gpu
+def test_invalid_sizes_batches_and_buffers_are_refused(dev):
+    import torch
+
+    import fft_demo_amd as F
+    from fft_demo_amd import _lib
+
+    lib = _lib.load()
+    d = _lib.Desc()
+    assert lib.mfft_desc_default(ctypes.byref(d)) == 0
+    h = ctypes.c_void_p()
+    # sizes with a prime factor the planner has no stage for, degenerate sizes, empty batches
+    for n, b, code in [(7 * 64, 4, ERR_UNSUPPORTED), (1, 4, None), (0, 4, ERR_INVALID), (256, 0, ERR_INVALID),
+                       (256, -1, ERR_INVALID)]:
+        rc = lib.mfft_plan_create_1d(ctypes.byref(h), n, b, ctypes.byref(d))
+        assert rc != 0 and not h.value, (n, b, rc)
+        if code is not None:
+            assert rc == code, (n, b, rc)
+        assert lib.mfft_last_error(), (n, b)
+    rc = lib.mfft_plan_create_2d(ctypes.byref(h), 7 * 32, 64, 2, ctypes.byref(d))
+    assert rc == ERR_UNSUPPORTED and not h.value
+    # exec on a valid plan: partial batches outside 1..batch and null buffers
+    p = F.Plan((256,), 8)
+    x = p.empty_in(device=dev)
+    y = p.empty_out(device=dev)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    for b in (0, -3, 9):
+        rc = lib.mfft_exec_ex(p._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), b, None,
+                              ctypes.c_void_p(s))
+        assert rc == ERR_INVALID, (b, rc)
+        assert b"batch" in lib.mfft_last_error()
+    assert lib.mfft_exec(p._h, None, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(s)) == ERR_INVALID
+    assert lib.mfft_exec(p._h, ctypes.c_void_p(x.data_ptr()), None, ctypes.c_void_p(s)) == ERR_INVALID
+    assert lib.mfft_exec(None, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                         ctypes.c_void_p(s)) == ERR_INVALID
+    # the plan still works after the refused calls
+    x.copy_(torch.ones_like(x))
+    p.exec(x, y)
+    torch.cuda.synchronize()
+    assert torch.isfinite(y.float()).all()
+    p.close()
