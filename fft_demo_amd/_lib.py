@@ -150,7 +150,7 @@ class ConvInfo(ctypes.Structure):
 
 CONV_SAME, CONV_VALID_CORR = 0, 1
 REAL_F32, REAL_F64 = 0, 1
-CONV_SPEC_FP32, CONV_SPEC_FP16, CONV_SPEC_FP32_MID_FP16 = 0, 1, 2
+CONV_SPEC_FP32, CONV_SPEC_FP16, CONV_SPEC_FP32_MID_FP16, CONV_SPEC_REFERENCE = 0, 1, 2, 3
 
 SIGNATURES.update({
     "mfft_conv2d_desc_default": (ctypes.c_int, [ctypes.POINTER(ConvDesc)]),

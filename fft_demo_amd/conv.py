@@ -15,10 +15,11 @@ MODES = {"same": L.CONV_SAME, "valid_corr": L.CONV_VALID_CORR}
 
 class Conv2d:
     def __init__(self, batch, channels, filters, h, w, fh, fw, mode="same", pad=None, precision="split_fp16",
-                 workspace_limit=0, spec_fp16=False, mid_fp16=False):
+                 workspace_limit=0, spec_fp16=False, mid_fp16=False, reference_method=False):
         """spec_fp16: fp16 image spectra and intermediates; mid_fp16: fp32 image spectra with
-        fp16 pass boundaries inside each transform (one channel, one filter per forward call).
-        Both serve forward calls only."""
+        fp16 pass boundaries inside each transform (one channel, one filter per forward call);
+        reference_method: the reference's fp16 method with per-map power-of-ten quantisation
+        of the product spectra. All three serve forward calls only."""
         lib = L.load()
         d = L.ConvDesc()
         lib.mfft_conv2d_desc_default(ctypes.byref(d))
@@ -28,9 +29,10 @@ class Conv2d:
             d.pad_h, d.pad_w = (pad, pad) if isinstance(pad, int) else pad
         d.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         d.workspace_limit = workspace_limit
-        if spec_fp16 and mid_fp16:
-            raise ValueError("spec_fp16 and mid_fp16 exclude each other")
-        d.spec_fp16 = L.CONV_SPEC_FP16 if spec_fp16 else L.CONV_SPEC_FP32_MID_FP16 if mid_fp16 else L.CONV_SPEC_FP32
+        if spec_fp16 + mid_fp16 + reference_method > 1:
+            raise ValueError("spec_fp16, mid_fp16 and reference_method exclude each other")
+        d.spec_fp16 = (L.CONV_SPEC_FP16 if spec_fp16 else L.CONV_SPEC_FP32_MID_FP16 if mid_fp16
+                       else L.CONV_SPEC_REFERENCE if reference_method else L.CONV_SPEC_FP32)
         h_ = ctypes.c_void_p()
         L.check(lib.mfft_conv2d_create(ctypes.byref(h_), ctypes.byref(d)), "mfft_conv2d_create")
         self._h = h_

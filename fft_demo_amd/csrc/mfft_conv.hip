@@ -5,8 +5,9 @@
 // (fft_conv2d_inverse_filter :1234-1253), real->complex padding with digit reversal
 // (fft_conv2d_bit_rev_real_pad_to_cpx_double2half :1222-1232), the channel sum
 // (fft_cpx_point_wise_mul_add :1033-1056), the per-filter power-of-10 quantisation
-// (quant_p1/p2/p3 :1058-1098) -- not needed here: spectra stay fp32 and the forward
-// transform is scaled per stage -- and the crop/scale readout
+// (quant_p1/p2/p3 :1058-1098; only the reference-method mode uses it -- otherwise spectra
+// stay fp32 or the transforms are scaled per stage), the dequantisation
+// (fft_cpx_point_wise_dequant :1129-1139) and the crop/scale readout
 // (read_out_scale_double2double_kernel :1390-1397).
 #include "mfft_common.h"
 
@@ -78,7 +79,7 @@ __global__ void pack_filters_kernel(const T *src, float2 *dst, int fh, int fw, i
 }
 
 // Y[k][f][p] = sum_c X[k][c][p] * H[f][c][p]   (X, Y: FMT; H: fp32)
-template <int FMT>
+template <int XFMT, int YFMT>
 __global__ void pointwise_mac_kernel(const void *X, const float2 *H, void *Y, int C, int F, long long P,
                                      long long total)
 {
@@ -90,12 +91,68 @@ __global__ void pointwise_mac_kernel(const void *X, const float2 *H, void *Y, in
     long long k = r / F;
     float re = 0.f, im = 0.f;
     for (int c = 0; c < C; ++c) {
-        float2 a = spec_ld<FMT>(X, (k * C + c) * P + p);
+        float2 a = spec_ld<XFMT>(X, (k * C + c) * P + p);
         float2 b = H[((long long)f * C + c) * P + p];
         re += a.x * b.x - a.y * b.y;
         im += a.x * b.y + a.y * b.x;
     }
-    spec_st<FMT>(Y, i, make_float2(re, im));
+    spec_st<YFMT>(Y, i, make_float2(re, im));
+}
+
+// The reference method's quantisation of the product spectra (quant_p1/p2/p3,
+// tensorcore_fft_conv2d_kernel.cu:1058-1098): per output map (pair, filter) the largest
+// |re| or |im| m over its P values sets the scale s = 1 if m <= 1000, else 10^(trunc(log10 m)
+// - 1), and the map is stored as fp16(R / s) for the fp16 inverse transform. Step 1: each
+// workgroup takes a chunk of one map and writes the chunk's maximum.
+__global__ void quant_max_kernel(const float2 *R, float *partial, long long P, int nchunk)
+{
+    __shared__ float red[256];
+    const long long map = blockIdx.x / nchunk;
+    const int chunk = (int)(blockIdx.x - map * nchunk);
+    const long long per = (P + nchunk - 1) / nchunk;
+    const long long b = chunk * per, e = b + per < P ? b + per : P;
+    float m = 0.f;
+    for (long long i = b + threadIdx.x; i < e; i += 256) {
+        const float2 v = R[map * P + i];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// The quantisation scale of a map from its largest magnitude (quant_p2's rule; the power of
+// ten by repeated products in fp32, exact up to 10^10).
+__device__ __forceinline__ float quant_scale(float m)
+{
+    if (!(m > 1000.f)) return 1.f;
+    const int k = (int)log10f(m) - 1;
+    float s = 1.f;
+    for (int j = 0; j < k; ++j) s *= 10.f;
+    return s;
+}
+
+// Step 2: each workgroup reduces its map's chunk maxima, takes the scale (chunk 0 records
+// it for the dequantisation) and stores its chunk as fp16(R / s).
+__global__ void quant_apply_kernel(const float2 *R, uint32_t *Q, const float *partial, float *scale, long long P,
+                                   int nchunk)
+{
+    const long long map = blockIdx.x / nchunk;
+    const int chunk = (int)(blockIdx.x - map * nchunk);
+    float m = 0.f;
+    for (int c = 0; c < nchunk; ++c) m = fmaxf(m, partial[map * nchunk + c]);
+    const float s = quant_scale(m), inv = 1.f / s;
+    if (chunk == 0 && threadIdx.x == 0) scale[map] = s;
+    const long long per = (P + nchunk - 1) / nchunk;
+    const long long b = chunk * per, e = b + per < P ? b + per : P;
+    for (long long i = b + threadIdx.x; i < e; i += 256) {
+        const float2 v = R[map * P + i];
+        Q[map * P + i] = pack_c16(v.x * inv, v.y * inv);
+    }
 }
 
 // out[2k][f][y][x] = scale * Re Y[k][f][y + oy][x + ox], out[2k+1][f][y][x] = scale * Im ...
@@ -105,7 +162,7 @@ __global__ void pointwise_mac_kernel(const void *X, const float2 *H, void *Y, in
 // rows whose width is a multiple of 4 on a 16-byte aligned buffer).
 template <typename T, int FMT, int V>
 __global__ void crop_unpack_kernel(const void *Y, T *out, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
-                                   int ox, float scale, int xb)
+                                   int ox, float scale, int xb, const float *map_scale)
 {
     const unsigned row = blockIdx.x / (unsigned)xb;
     const int x = ((int)(blockIdx.x - row * (unsigned)xb) * 256 + threadIdx.x) * V;
@@ -117,6 +174,7 @@ __global__ void crop_unpack_kernel(const void *Y, T *out, int B, int F, int oh, 
     const long long src = ((k * F + f) * Ph + (y + oy)) * (long long)Pw + (x + ox);
     const long long plane = (long long)F * oh * ow, off = ((long long)f * oh + y) * ow + x;
     const bool second = 2 * k + 1 < B;
+    if (map_scale) scale *= map_scale[k * F + f];  // the reference method's dequantisation
     if constexpr (V == 1) {
         const float2 v = spec_ld<FMT>(Y, src);
         out[2 * k * plane + off] = (T)(v.x * scale);
@@ -308,21 +366,37 @@ hipError_t conv_pack_filters(const void *src, int f64, float2 *dst, int FC, int 
 }
 
 hipError_t conv_pointwise(const void *X, const float2 *H, void *Y, int fmt, long long pairs, int C, int F, long long P,
-                          hipStream_t s)
+                          hipStream_t s, int yfmt)
 {
     long long total = pairs * F * P;
-    if (fmt == FMT_C16)
-        hipLaunchKernelGGL(pointwise_mac_kernel<FMT_C16>, dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C, F, P,
-                           total);
+    if (yfmt < 0) yfmt = fmt;
+    if (fmt == FMT_C16 && yfmt == FMT_C16)
+        hipLaunchKernelGGL((pointwise_mac_kernel<FMT_C16, FMT_C16>), dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C,
+                           F, P, total);
+    else if (fmt == FMT_C16)
+        hipLaunchKernelGGL((pointwise_mac_kernel<FMT_C16, FMT_C32>), dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C,
+                           F, P, total);
     else
-        hipLaunchKernelGGL(pointwise_mac_kernel<FMT_C32>, dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C, F, P,
-                           total);
+        hipLaunchKernelGGL((pointwise_mac_kernel<FMT_C32, FMT_C32>), dim3(nblocks(total)), dim3(256), 0, s, X, H, Y, C,
+                           F, P, total);
+    return hipGetLastError();
+}
+
+hipError_t conv_quantise(const float2 *R, void *Q, float *partial, float *scale, long long maps, long long P,
+                         int nchunk, hipStream_t s)
+{
+    const long long blocks = maps * nchunk;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(quant_max_kernel, dim3((unsigned)blocks), dim3(256), 0, s, R, partial, P, nchunk);
+    hipLaunchKernelGGL(quant_apply_kernel, dim3((unsigned)blocks), dim3(256), 0, s, R, (uint32_t *)Q, partial, scale, P,
+                       nchunk);
     return hipGetLastError();
 }
 
 template <int FMT>
 static hipError_t launch_crop(const void *Y, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
-                              int ox, float scale, hipStream_t s)
+                              int ox, float scale, hipStream_t s, const float *map_scale)
 {
     const bool vec = !f64 && ow % 4 == 0 && ((uintptr_t)out & 15) == 0;
     const int xb = (int)nblocks(vec ? (ow + 3) / 4 : ow);
@@ -331,21 +405,21 @@ static hipError_t launch_crop(const void *Y, void *out, int f64, int B, int F, i
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     if (f64)
         hipLaunchKernelGGL((crop_unpack_kernel<double, FMT, 1>), dim3((unsigned)blocks), dim3(256), 0, s, Y,
-                           (double *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb);
+                           (double *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, map_scale);
     else if (vec)
         hipLaunchKernelGGL((crop_unpack_kernel<float, FMT, 4>), dim3((unsigned)blocks), dim3(256), 0, s, Y,
-                           (float *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb);
+                           (float *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, map_scale);
     else
         hipLaunchKernelGGL((crop_unpack_kernel<float, FMT, 1>), dim3((unsigned)blocks), dim3(256), 0, s, Y,
-                           (float *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb);
+                           (float *)out, B, F, oh, ow, Ph, Pw, oy, ox, scale, xb, map_scale);
     return hipGetLastError();
 }
 
 hipError_t conv_crop(const void *Y, int fmt, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
-                     int ox, float scale, hipStream_t s)
+                     int ox, float scale, hipStream_t s, const float *map_scale)
 {
-    if (fmt == FMT_C16) return launch_crop<FMT_C16>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, s);
-    return launch_crop<FMT_C32>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, s);
+    if (fmt == FMT_C16) return launch_crop<FMT_C16>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, s, map_scale);
+    return launch_crop<FMT_C32>(Y, out, f64, B, F, oh, ow, Ph, Pw, oy, ox, scale, s, map_scale);
 }
 
 hipError_t conv_pointwise_gw(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph, int Pw,

@@ -26,9 +26,11 @@ hipError_t conv_pack_pairs(const void *src, int f64, void *dst, int fmt, int B, 
 hipError_t conv_pack_filters(const void *src, int f64, float2 *dst, int FC, int fh, int fw, int Ph, int Pw, int rotate,
                              hipStream_t s);
 hipError_t conv_pointwise(const void *X, const float2 *H, void *Y, int fmt, long long pairs, int C, int F, long long P,
-                          hipStream_t s);
+                          hipStream_t s, int yfmt = -1);
+hipError_t conv_quantise(const float2 *R, void *Q, float *partial, float *scale, long long maps, long long P,
+                         int nchunk, hipStream_t s);
 hipError_t conv_crop(const void *Y, int fmt, void *out, int f64, int B, int F, int oh, int ow, int Ph, int Pw, int oy,
-                     int ox, float scale, hipStream_t s);
+                     int ox, float scale, hipStream_t s, const float *map_scale = nullptr);
 hipError_t conv_pointwise_gw(const float2 *X, const float2 *D, float2 *G, int pairs, int C, int F, int Ph, int Pw,
                              hipStream_t s);
 hipError_t conv_pointwise_gi(const float2 *D, const float2 *H, float2 *R, long long pairs, int C, int F, long long P,
@@ -54,6 +56,12 @@ struct mfft_conv2d_s {
     // run out of place, so a transform whose input is not fused reads Y and writes X, and the
     // inverse of a pointwise product in Y writes X
     bool mid16 = false;
+    // the reference's method (MFFT_CONV_SPEC_REFERENCE): unscaled fp16 transforms, the fp32
+    // product spectra in Y quantised per output map into Q (fp16) with the scales in qscale
+    bool refq = false;
+    void *Q = nullptr;
+    float *qscale = nullptr, *qpart = nullptr;
+    int qchunks = 1;
     mfft_plan fwd_img = nullptr, fwd_filt = nullptr, inv = nullptr;
     float2 *H = nullptr, *X = nullptr, *Y = nullptr;
     // backward (built on its first call): delta transform, inverses over (filter, channel)
@@ -148,6 +156,9 @@ int mfft_conv2d_destroy(mfft_conv2d op)
     (void)hipFree(op->H);
     (void)hipFree(op->X);
     (void)hipFree(op->Y);
+    (void)hipFree(op->Q);
+    (void)hipFree(op->qscale);
+    (void)hipFree(op->qpart);
     op->order.destroy();
     delete op;
     return MFFT_OK;
@@ -162,13 +173,14 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     if (d->mode != MFFT_CONV_SAME && d->mode != MFFT_CONV_VALID_CORR) return cfail(MFFT_ERR_INVALID, "mode");
     if (d->mode == MFFT_CONV_VALID_CORR && (d->fh > d->h || d->fw > d->w))
         return cfail(MFFT_ERR_INVALID, "filter larger than the image in 'valid' mode");
-    if (d->spec_fp16 < MFFT_CONV_SPEC_FP32 || d->spec_fp16 > MFFT_CONV_SPEC_FP32_MID_FP16)
+    if (d->spec_fp16 < MFFT_CONV_SPEC_FP32 || d->spec_fp16 > MFFT_CONV_SPEC_REFERENCE)
         return cfail(MFFT_ERR_INVALID, "spec_fp16");
     if (d->spec_fp16 == MFFT_CONV_SPEC_FP32_MID_FP16 && d->channels != 1)
         return cfail(MFFT_ERR_UNSUPPORTED, "fp16 pass boundaries need a single-channel operator");
     mfft_conv2d op = new mfft_conv2d_s;
     op->d = *d;
-    op->spec_fmt = d->spec_fp16 == MFFT_CONV_SPEC_FP16 ? MFFT_FMT_C16 : MFFT_FMT_C32;
+    op->refq = d->spec_fp16 == MFFT_CONV_SPEC_REFERENCE;
+    op->spec_fmt = d->spec_fp16 == MFFT_CONV_SPEC_FP16 || op->refq ? MFFT_FMT_C16 : MFFT_FMT_C32;
     op->mid16 = d->spec_fp16 == MFFT_CONV_SPEC_FP32_MID_FP16;
     int need_h = d->h + d->fh - 1, need_w = d->w + d->fw - 1;
     if (d->mode == MFFT_CONV_VALID_CORR) {
@@ -201,6 +213,13 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     if (op->spec_fmt == MFFT_FMT_C16 || op->mid16) fd.mid_fmt = MFFT_FMT_C16;
     fd.workspace_limit = d->workspace_limit;
     fd.scale = MFFT_SCALE_STAGE;  // image spectra carry 1/(Ph*Pw): the inverse is then unit
+    if (op->refq) {
+        // the reference's transforms: fp16 operands, no scaling in either direction
+        // (tensorcore_fft_conv2d_kernel.cu:109-1031); 1/(Ph*Pw) and the quantisation scale
+        // are applied by the crop
+        fd.precision = MFFT_PREC_FP16;
+        fd.scale = MFFT_SCALE_NONE;
+    }
     // fp16 pass boundaries with fp32 ends: the intermediates the passes keep in the output
     // buffer are fp16 as well (the image transforms then never run in place)
     auto create = [&](mfft_plan *pl, int64_t b) {
@@ -228,13 +247,25 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     long long P = (long long)op->Ph * op->Pw;
     op->spec_bytes = (long long)d->filters * d->channels * P * 8;
     const long long eb = op->spec_fmt == MFFT_FMT_C16 ? 4 : 8;
-    long long xb = op->pairs * d->channels * P * eb, yb = op->pairs * d->filters * P * eb;
+    const long long maps = op->pairs * d->filters;
+    long long xb = op->pairs * d->channels * P * eb, yb = maps * P * (op->refq ? 8 : eb);
     if (hipMalloc(&op->H, op->spec_bytes) != hipSuccess || hipMalloc(&op->X, xb) != hipSuccess ||
         hipMalloc(&op->Y, yb) != hipSuccess) {
         mfft_conv2d_destroy(op);
         return cfail(MFFT_ERR_NOMEM, "conv buffers (%lld + %lld + %lld bytes)", op->spec_bytes, xb, yb);
     }
     op->bytes = op->spec_bytes + xb + yb;
+    if (op->refq) {
+        // quantised product spectra, one scale per map, chunk maxima (one chunk per 16K values)
+        op->qchunks = (int)std::max(1LL, std::min(256LL, P / 16384));
+        const long long qb = maps * P * 4, sb = maps * 4, pb = maps * op->qchunks * 4;
+        if (hipMalloc(&op->Q, qb) != hipSuccess || hipMalloc((void **)&op->qscale, sb) != hipSuccess ||
+            hipMalloc((void **)&op->qpart, pb) != hipSuccess) {
+            mfft_conv2d_destroy(op);
+            return cfail(MFFT_ERR_NOMEM, "quantisation buffers (%lld bytes)", qb + sb + pb);
+        }
+        op->bytes += qb + sb + pb;
+    }
     mfft_info inf;
     for (mfft_plan p : {op->fwd_img, op->fwd_filt, op->inv})
         if (!mfft_plan_info(p, &inf)) op->bytes += inf.workspace_bytes;
@@ -297,6 +328,26 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
     const int nf = fsel >= 0 ? 1 : d.filters;
     const float2 *H = fsel >= 0 ? op->H + (long long)fsel * d.channels * P : op->H;
     float2 *res = op->Y;  // the inverse's output
+    if (op->refq) {
+        // the reference's method: channel sum in fp32, each output map quantised to fp16 with
+        // its power-of-ten scale, unscaled fp16 inverse, dequantised by the crop
+        mfft::trace_launch(s, "conv_pointwise", op->pairs * d.channels * P * 4 + (long long)nf * d.channels * P * 8,
+                           op->pairs * nf * P * 8);
+        if (mfft::conv_pointwise(op->X, H, op->Y, MFFT_FMT_C16, op->pairs, d.channels, nf, P, s, MFFT_FMT_C32) !=
+            hipSuccess)
+            return cfail(MFFT_ERR_HIP, "pointwise");
+        mfft::trace_launch(s, "conv_quantise", 2 * op->pairs * nf * P * 8, op->pairs * nf * P * 4);
+        if (mfft::conv_quantise(op->Y, op->Q, op->qpart, op->qscale, op->pairs * nf, P, op->qchunks, s) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "quantise");
+        rc = mfft_exec_ex(op->inv, op->Q, op->Q, op->pairs * nf, nullptr, s);
+        if (rc) return cfail(rc, "%s", mfft_last_error());
+        mfft::trace_launch(s, "conv_crop", op->pairs * nf * (long long)op->oh * op->ow * 4,
+                           (long long)d.batch * nf * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
+        if (mfft::conv_crop(op->Q, MFFT_FMT_C16, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
+                            op->Pw, op->oy, op->ox, (float)(1.0 / (double)P), s, op->qscale) != hipSuccess)
+            return cfail(MFFT_ERR_HIP, "crop");
+        return MFFT_OK;
+    }
     if (d.channels == 1 && nf == 1 && mfft::plan_first_pass_fusable(op->inv)) {
         // one filter, one channel: the inverse's first pass multiplies by the spectrum
         mfft::ExecFuse f;
@@ -339,6 +390,7 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
     if (!op->have_filters) return cfail(MFFT_ERR_INVALID, "filters not set");
     const mfft_conv2d_desc &d = op->d;
     if (d.channels != 1) return cfail(MFFT_ERR_UNSUPPORTED, "forward_two needs a single-channel operator");
+    if (op->refq) return cfail(MFFT_ERR_UNSUPPORTED, "forward_two: not in the reference-method mode");
     if (filter0 < 0 || filter1 < 0 || filter0 >= d.filters || filter1 >= d.filters)
         return cfail(MFFT_ERR_INVALID, "filter index");
     hipStream_t s = (hipStream_t)stream;

@@ -215,7 +215,7 @@ int mfft_stage_convert(const void *src, int src_fmt, void *dst, int dst_fmt, int
  * ------------------------------------------------------------------------------------ */
 enum { MFFT_CONV_SAME = 0, MFFT_CONV_VALID_CORR = 1 };
 enum { MFFT_REAL_F32 = 0, MFFT_REAL_F64 = 1 };
-enum { MFFT_CONV_SPEC_FP32 = 0, MFFT_CONV_SPEC_FP16 = 1, MFFT_CONV_SPEC_FP32_MID_FP16 = 2 };
+enum { MFFT_CONV_SPEC_FP32 = 0, MFFT_CONV_SPEC_FP16 = 1, MFFT_CONV_SPEC_FP32_MID_FP16 = 2, MFFT_CONV_SPEC_REFERENCE = 3 };
 
 typedef struct mfft_conv2d_desc {
     int batch, channels, filters;
@@ -229,8 +229,16 @@ typedef struct mfft_conv2d_desc {
                            the outputs about 1e-4..5e-4). FP32_MID_FP16 (2): fp32 image spectra
                            and transform outputs, fp16 words at the pass boundaries inside each
                            transform; single-channel operators, one filter per call (filters
-                           = 1 or mfft_conv2d_forward_select). FP16 and FP32_MID_FP16 serve
-                           forward calls only: mfft_conv2d_backward needs FP32. */
+                           = 1 or mfft_conv2d_forward_select). REFERENCE (3): the reference's
+                           method (tensorcore_fft_conv2d_kernel.cu:1033-1150): fp16 transforms
+                           without scaling, fp16 image spectra, the channel sum in fp32, each
+                           output map quantised to fp16 with a power-of-ten scale (1 if its
+                           largest |re|, |im| is at most 1000, else 10^(trunc(log10 max) - 1))
+                           before the fp16 inverse, dequantised with the 1/(Ph*Pw) readout; the
+                           precision field is ignored. Like the reference, the unscaled fp16
+                           forward overflows once an image's padded sum exceeds 65504 (256^2
+                           images of [0,1] values fit). Only FP32 serves mfft_conv2d_backward, and
+                           forward_two takes FP32, FP16 or FP32_MID_FP16. */
     int64_t workspace_limit;
 } mfft_conv2d_desc;
 

@@ -18,7 +18,7 @@ def test_conv_create_errors_reach_last_error():
     assert lib.mfft_conv2d_create(ctypes.byref(h), ctypes.byref(d)) == ERR_INVALID
     assert b"bad sizes" in lib.mfft_last_error()
     d.h, d.w, d.fh, d.fw = 32, 32, 3, 3
-    d.spec_fp16 = 3
+    d.spec_fp16 = 4
     assert lib.mfft_conv2d_create(ctypes.byref(h), ctypes.byref(d)) == ERR_INVALID
     assert b"spec_fp16" in lib.mfft_last_error()
     # fp32 spectra with fp16 pass boundaries: single-channel operators only

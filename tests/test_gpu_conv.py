@@ -260,6 +260,48 @@ def test_conv_forward_two_fp32_spectra_fp16_boundaries(dev, pad):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,C,F,h,w,fh,fw,mode", [(1, 1, 16, 256, 256, 13, 13, "valid_corr"),
+                                                  (3, 2, 2, 64, 80, 7, 5, "same")])
+def test_conv_reference_method(dev, B, C, F, h, w, fh, fw, mode):
+    """The reference's method (reference_method=True; tensorcore_fft_conv2d_kernel.cu:1033-1150):
+    unscaled fp16 transforms, fp16 image spectra, channel sum in fp32, each output map
+    quantised to fp16 with its power-of-ten scale, dequantised at the readout. First the
+    reference's accuracy_conv shape (main.cpp:320-352: 256^2, 16 filters of 13^2, 'valid'
+    correlation), then several channels and an odd batch in 'same' mode, against the FP64
+    oracle; the fp16 spectra bound the error (rel-L2 <= 5e-3, rel-max <= 5e-2 of each map).
+    forward_two and the backward pass are refused in this mode."""
+    import torch
+
+    from fft_demo_amd.conv import Conv2d
+
+    img = oracle.uniform(B * C * h * w, 101, 0.0, 1.0).reshape(B, C, h, w)
+    filt = oracle.uniform(F * C * fh * fw, 102, -1.0, 1.0).reshape(F, C, fh, fw)
+    op = Conv2d(B, C, F, h, w, fh, fw, mode=mode, reference_method=True)
+    op.set_filters(torch.from_numpy(filt).to(device=dev, dtype=torch.float32).contiguous())
+    x = torch.from_numpy(img).to(device=dev, dtype=torch.float32).contiguous()
+    got = op.forward(x).double().cpu().numpy()
+    worst = (0.0, 0.0)
+    for b in range(B):
+        for f in range(F):
+            if mode == "same":
+                ref = sum(oracle.conv2d_same(img[b, c], filt[f, c]) for c in range(C))
+            else:
+                ref = sum(oracle.corr2d_valid(img[b, c], filt[f, c]) for c in range(C))
+            l2, mx = oracle.err_l2max(got[b, f], ref)
+            worst = (max(worst[0], l2), max(worst[1], mx))
+            assert l2 <= 5e-3 and mx <= 5e-2, (b, f, l2, mx)
+    print(f"reference method {mode} {h}x{w} pad={op.pad} C={C} F={F}: worst rel_l2={worst[0]:.2e} "
+          f"rel_max={worst[1]:.2e}")
+    if C == 1:
+        with pytest.raises(RuntimeError, match="reference-method"):
+            op.forward_two(x[0, 0], x[0, 0], 0, 0)
+    if mode == "valid_corr":
+        with pytest.raises(RuntimeError, match="fp32 spectra"):
+            op.backward(torch.zeros(B, F, h - fh + 1, w - fw + 1, device=dev), x)
+    op.close()
+
+
+@pytest.mark.gpu
 def test_conv_forward_two_fp16_spectra_and_backward_refused(dev):
     """forward_two with fp16 spectra against the oracle (tolerance as for spec_fp16 above);
     the backward pass of such an operator is refused (it needs fp32 spectra)."""
