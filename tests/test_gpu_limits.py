@@ -108,3 +108,28 @@ def test_invalid_sizes_batches_and_buffers_are_refused(dev):
     torch.cuda.synchronize()
     assert torch.isfinite(y.float()).all()
     p.close()
+
+
+@pytest.mark.gpu
+def test_batch_chunks_are_equal(dev):
+    """A workspace limit that holds 7 of 8 transforms runs the batch as two chunks of 4 (one
+    chunk of 7 and one of 1 would leave most of the device idle on the second); the results
+    equal those of a plan whose workspace holds the whole batch, bit for bit (the same kernels
+    per transform)."""
+    import torch
+
+    import fft_demo_amd as F
+    from tests.helpers import complex_input, to_device
+
+    n, b = 65536, 8
+    per = n * 4  # one transform's 16-bit intermediate
+    small = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16", split_hint=256, workspace_limit=7 * per)
+    whole = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16", split_hint=256, workspace_limit=16 * per)
+    assert small.info["batch_chunk"] == 4 and whole.info["batch_chunk"] == 8
+    assert small.info["launches_per_exec"] == 2 * len(small.info["passes"])
+    x = to_device(complex_input((b, n), 31), small.in_dtype(), dev)
+    y1, y2 = small.exec(x), whole.exec(x)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    small.close()
+    whole.close()
