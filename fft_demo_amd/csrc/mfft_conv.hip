@@ -126,10 +126,12 @@ __global__ void quant_max_kernel(const float2 *R, float *partial, long long P, i
 }
 
 // The quantisation scale of a map from its largest magnitude (quant_p2's rule; the power of
-// ten by repeated products in fp32, exact up to 10^10).
+// ten by repeated products in fp32, exact up to 10^10). A map whose forward overflowed (an
+// infinite or NaN maximum) keeps scale 1, so the overflow shows in its output as it does in
+// the reference's; the loop is bounded by the largest finite maximum (k <= 37).
 __device__ __forceinline__ float quant_scale(float m)
 {
-    if (!(m > 1000.f)) return 1.f;
+    if (!(m > 1000.f) || !(m <= 3.4e38f)) return 1.f;
     const int k = (int)log10f(m) - 1;
     float s = 1.f;
     for (int j = 0; j < k; ++j) s *= 10.f;
