@@ -56,6 +56,7 @@ struct mfft_conv2d_s {
     // run out of place, so a transform whose input is not fused reads Y and writes X, and the
     // inverse of a pointwise product in Y writes X
     bool mid16 = false;
+    int res_fmt = MFFT_FMT_C32;  // the inverse transform's output, which the crop reads
     // the reference's method (MFFT_CONV_SPEC_REFERENCE): unscaled fp16 transforms, the fp32
     // product spectra in Y quantised per output map into Q (fp16) with the scales in qscale
     bool refq = false;
@@ -230,8 +231,13 @@ int mfft_conv2d_create(mfft_conv2d *out, const mfft_conv2d_desc *d)
     if (!rc) {
         fd.direction = MFFT_INVERSE;
         fd.scale = MFFT_SCALE_NONE;
+        // fp16 pass boundaries: the inverse's output, read only by the crop, is fp16 words as
+        // well (the spectra it reads stay fp32)
+        if (op->mid16) fd.out_fmt = MFFT_FMT_C16;
         rc = create(&op->inv, op->pairs * d->filters);
+        fd.out_fmt = op->spec_fmt;
     }
+    op->res_fmt = op->mid16 ? MFFT_FMT_C16 : op->spec_fmt;
     if (!rc) {
         // the resident filter spectra stay fp32 whatever the image spectra
         fd.direction = MFFT_FORWARD;
@@ -363,9 +369,9 @@ static int conv_run(mfft_conv2d op, const void *in, int in_fmt, void *out, int o
         rc = mfft_exec_ex(op->inv, op->Y, res, op->pairs * nf, nullptr, s);
     }
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    mfft::trace_launch(s, "conv_crop", op->pairs * nf * (long long)op->oh * op->ow * spec_eb(op),
+    mfft::trace_launch(s, "conv_crop", op->pairs * nf * (long long)op->oh * op->ow * (op->res_fmt == MFFT_FMT_C16 ? 4 : 8),
                        (long long)d.batch * nf * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
-    if (mfft::conv_crop(res, op->spec_fmt, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
+    if (mfft::conv_crop(res, op->res_fmt, out, out_fmt == MFFT_REAL_F64, d.batch, nf, op->oh, op->ow, op->Ph,
                         op->Pw, op->oy, op->ox, 1.0f, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
@@ -424,9 +430,9 @@ int mfft_conv2d_forward_two(mfft_conv2d op, const void *in0, const void *in1, in
     float2 *res = op->mid16 ? op->X : op->Y;  // mid16: the inverse runs out of place
     rc = mfft_exec_ex(op->inv, op->Y, res, 1, nullptr, s);
     if (rc) return cfail(rc, "%s", mfft_last_error());
-    mfft::trace_launch(s, "conv_crop_two", (long long)op->oh * op->ow * spec_eb(op),
+    mfft::trace_launch(s, "conv_crop_two", (long long)op->oh * op->ow * (op->res_fmt == MFFT_FMT_C16 ? 4 : 8),
                        2LL * op->oh * op->ow * (out_fmt == MFFT_REAL_F64 ? 8 : 4));
-    if (mfft::conv_crop_two(res, op->spec_fmt, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy,
+    if (mfft::conv_crop_two(res, op->res_fmt, out0, out1, out_fmt == MFFT_REAL_F64, op->oh, op->ow, op->Pw, op->oy,
                             op->ox, s) != hipSuccess)
         return cfail(MFFT_ERR_HIP, "crop");
     return MFFT_OK;
