@@ -787,7 +787,8 @@ def conv_config(steps, warmup, with_cpu=True):
     P = head["pad"][0] * head["pad"][1]
     outd = {"workload": "FFT 'same' conv2d, 2048x2048 fp32 x 16, 31x31 Gaussian filter, pad 2176, split-fp16 "
                         "transforms, fp32 image spectra, fp16 words at the pass boundaries inside each transform "
-                        "(as the 1D and 2D workloads), resident fp32 filter spectrum",
+                        "and from the inverse to the crop (as the 1D and 2D workloads), resident fp32 filter "
+                        "spectrum",
             "value": head["images_per_s"], "unit": "images/s", "ms_per_batch": head["ms_per_batch"],
             "steps": steps, "nominal_gflops": round(B * 2 * 5.0 * P * math.log2(P) / (head["ms_per_batch"] * 1e-3) / 1e9, 1),
             "pad": head["pad"], "rel_l2": head["rel_l2"], "rel_max": head["rel_max"],
