@@ -32,6 +32,23 @@ enum { KIND_ROWS = 0, KIND_ROWS_TR = 1, KIND_COLS = 2, KIND_ONCHIP = 3, KIND_ONC
 constexpr int kColsMaxR = 1024;
 constexpr int cols_lines(int R) { return 8192 / R; }
 
+// Waves per workgroup (host and device). fp32: one 16-point column per thread in the radix-16
+// stages. Matrix-core precisions: the strided kind keeps two 16-column MFMA groups per wave
+// (16 waves for its 8192-point tiles); the row kinds keep one group per wave (16 waves) for
+// lines of 512 points or more and two (8 waves) below. Measured on 2^26-point batches, one
+// group per wave ran the row kernels 1-5% faster at 512..4096 points (the 1024^2 x 64 2D
+// exec 265 -> 254 us); at 256 points and 4096 lines (one tile per compute unit) it ran 24%
+// slower (4.3 -> 5.3 us per exec).
+constexpr int fast_waves_of(int prec, int R, int kind)
+{
+    const int T = kind == KIND_COLS ? cols_lines(R) : kMaxTile / R;
+    const int groups = R * T / 256;
+    if (prec == PREC_FP32) return R * T / 16 / 64;
+    const int per_wave = (kind != KIND_COLS && R >= 512) ? 1 : 2;
+    const int w = groups / per_wave;
+    return w < 4 ? 4 : (w > 16 ? 16 : w);
+}
+
 // Precisions and input formats whose fast kernels also read digit-reversed lines
 // (TileArgs::rev_input): the matrix-core paths with fp16 or fp32 input.
 constexpr bool fast_rev_ok(int prec, int in_fmt) { return prec != PREC_FP32 && in_fmt != FMT_CBF16; }

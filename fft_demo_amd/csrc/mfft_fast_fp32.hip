@@ -14,12 +14,5 @@ fast_fn find_fast_kernel(int prec, int R, int kind, int inf, int outf)
 }
 
 int fast_tile_lines(int R, int kind) { return kind == KIND_COLS ? cols_lines(R) : kMaxTile / R; }
-int fast_threads(int prec, int R, int kind)
-{
-    // the host side of fast_waves<PREC, KIND, R>()
-    const int T = fast_tile_lines(R, kind), groups = R * T / 256;
-    if (prec == PREC_FP32) return R * T / 16;
-    const int w = groups / 2 < 4 ? 4 : (groups / 2 > 16 ? 16 : groups / 2);
-    return w * 64;
-}
+int fast_threads(int prec, int R, int kind) { return fast_waves_of(prec, R, kind) * 64; }
 }  // namespace mfft

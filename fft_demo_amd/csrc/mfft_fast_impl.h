@@ -48,15 +48,12 @@ struct Elem<FMT_C32> {
 };
 
 // ------------------------------------------------------------------ geometry
-// Waves per workgroup. A tile is T lines of R points (T = kMaxTile / R; cols_lines(R) for
-// the strided kind). fp32: one 16-point column per thread in the radix-16 stages. Matrix-core
-// precisions: two 16-column MFMA groups per wave, 4..16 waves.
+// Waves per workgroup (fast_waves_of). A tile is T lines of R points (T = kMaxTile / R;
+// cols_lines(R) for the strided kind).
 template <int PREC, int KIND, int R>
 constexpr int fast_waves()
 {
-    constexpr int T = KIND == KIND_COLS ? cols_lines(R) : kMaxTile / R;
-    constexpr int groups = R * T / 256;
-    return PREC == PREC_FP32 ? R * T / 16 / 64 : (groups / 2 < 4 ? 4 : (groups / 2 > 16 ? 16 : groups / 2));
+    return fast_waves_of(PREC, R, KIND);
 }
 
 template <int R, int KIND, int NW_ = 4>
