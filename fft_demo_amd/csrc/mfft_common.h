@@ -121,6 +121,11 @@ struct TileArgs {
     // butterflies, generated in FP64 on the GPU and rounded once to fp32 like every other
     // table (gen_w16)
     float w16[3];
+    // streaming kernel: bit 0 (1) every input byte offset of the launch, bit 1 (2) every
+    // output byte offset, is below 2^32; in_span / out_span: the bytes the launch reaches from
+    // in / out (the extent of the buffer resource its loads / stores then go through)
+    int off32;
+    unsigned in_span, out_span;
 };
 
 // DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of

@@ -1024,6 +1024,16 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         }
         if (ps.small) {
             long long nl = (long long)a.c0 * a.c1 * a.c2;
+            // largest byte offset a line's point reaches, per side (strides are >= 0)
+            auto span = [&](long long s0, long long s1, long long s2, long long sp, int fmt) -> long long {
+                if (s0 < 0 || s1 < 0 || s2 < 0 || sp < 0) return -1;
+                return ((a.c0 - 1) * s0 + (a.c1 - 1) * s1 + (a.c2 - 1) * s2 + (a.R - 1) * sp) * fmt_bytes(fmt);
+            };
+            const long long si = span(a.in_s0, a.in_s1, a.in_s2, a.in_ps, a.in_fmt);
+            const long long so = span(a.out_s0, a.out_s1, a.out_s2, a.out_ps, a.out_fmt);
+            a.off32 = (si >= 0 && si < (1LL << 32) - 8 ? 1 : 0) | (so >= 0 && so < (1LL << 32) - 8 ? 2 : 0);
+            a.in_span = (a.off32 & 1) ? (unsigned)(si + fmt_bytes(a.in_fmt)) : 0u;
+            a.out_span = (a.off32 & 2) ? (unsigned)(so + fmt_bytes(a.out_fmt)) : 0u;
             HIP_TRY(launch(dim3((unsigned)((nl + 255) / 256)), dim3(256)));
             continue;
         }
