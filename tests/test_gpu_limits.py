@@ -133,3 +133,45 @@ def test_batch_chunks_are_equal(dev):
     assert torch.equal(y1, y2)
     small.close()
     whole.close()
+
+
+@pytest.mark.gpu
+def test_streaming_pass_beyond_32bit_offsets(dev):
+    """The streaming kernel addresses through a buffer resource with 32-bit offsets when every
+    offset of its launch fits, and with 64-bit arithmetic otherwise. 2176-point lines (a
+    radix-17 streaming pass, then 128-point matrix-core passes) x 2^19 in one chunk reach
+    4.56 GB of fp16 words (above 2^32 bytes); the same plan in 128 MiB chunks stays below.
+    Both are checked on sampled lines against the oracle and against each other."""
+    import torch
+
+    import fft_demo_amd as F
+    import oracle
+    from tests.helpers import from_device, rel_errors
+
+    n, b = 2176, 1 << 19
+    one = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16", workspace_limit=8 << 30)
+    chunked = F.Plan((n,), b, precision="split_fp16", mid_fmt="c16")
+    print(f"passes {[(p['kernel'], p['R']) for p in one.info['passes']]}, chunks {one.info['batch_chunk']} / "
+          f"{chunked.info['batch_chunk']}", flush=True)
+    assert one.info["passes"][0]["kernel"] == "stream_small" and one.info["passes"][0]["R"] == 17
+    assert one.info["batch_chunk"] == b and chunked.info["batch_chunk"] * n * 4 <= 128 << 20
+    g = torch.Generator(device=dev).manual_seed(2176)
+    x = (torch.rand((b, n, 2), generator=g, device=dev) * 2 - 1).half()
+    y1, y2 = one.exec(x), chunked.exec(x)
+    torch.cuda.synchronize()
+    lines = [0, 1, 12345, b // 2 + 7, b - 2, b - 1]
+    xs = from_device(x[lines])
+    ref = oracle.fft(xs, sign=-1) * one.out_scale
+    for y in (y1, y2):
+        l2, mx = rel_errors(from_device(y[lines]), ref)
+        print(f"sampled lines vs oracle: rel_l2 {l2:.3e} rel_max {mx:.3e}", flush=True)
+        assert l2 <= 1e-3 and mx <= 4e-3
+    num = den = 0.0
+    for s in range(0, b, 1 << 16):
+        a1, a2 = y1[s:s + (1 << 16)].float(), y2[s:s + (1 << 16)].float()
+        num += torch.sum((a1 - a2) ** 2).item()
+        den += torch.sum(a2 ** 2).item()
+    print(f"one chunk vs 128 MiB chunks: rel_l2 {(num / den) ** 0.5:.3e}", flush=True)
+    assert (num / den) ** 0.5 <= 5e-4
+    for p in (one, chunked):
+        p.close()
