@@ -101,6 +101,26 @@ struct Origin {
     long long in_base, out_base;
 };
 
+// Row tiles of 256-point lines on the matrix cores: MFMA group P of a stage is line P in
+// both radix-16 stages (16 slots of a line per group) and P = wave + u NW, so every line's
+// LDS exchange, and the next tile's overwrite of it, stays inside one wave. LDS operations of
+// a wave execute in order, so such a tile needs no workgroup barrier between its stages
+// or before the next tile, only a point the compiler does not move LDS accesses across.
+template <int PREC, int R, int KIND>
+constexpr bool wave_lines() { return PREC != PREC_FP32 && KIND == KIND_ROWS && R == 256; }
+
+template <int PREC, int R, int KIND>
+__device__ __forceinline__ void tile_sync()
+{
+    if constexpr (wave_lines<PREC, R, KIND>()) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // The tiles of one workgroup form the progression tile0, tile0 + step, ... The walker keeps
 // the current tile as (t0, l1, l2) with tile = t0 + tiles0 (l1 + c1 l2), the step split the
 // same way, and the element offsets of the tile's first line; advancing adds the parts with
@@ -529,7 +549,7 @@ __device__ __forceinline__ void mstage(float2 *lds, const TileArgs &a, const Ori
             }
         }
     }
-    if constexpr (!LAST) __syncthreads();
+    if constexpr (!LAST) tile_sync<PREC, R, KIND>();
 }
 
 // ------------------------------------------------------------------ radix-16 stage (fp32 VALU)
@@ -851,7 +871,7 @@ __device__ __forceinline__ void fast_tiles(const TileArgs &a, float2 *lds, float
             else
                 mstage<PREC, R, KIND, MAP_OUT, false, true, INF, OUTF, TW, REV>(lds, a, o, s, op, cur, stw, s_tw);
         }
-        __syncthreads();  // the next tile's first stage overwrites LDS
+        tile_sync<PREC, R, KIND>();  // the next tile's first stage overwrites LDS
         tile = next;
         o = on;
         cur = nxt;
