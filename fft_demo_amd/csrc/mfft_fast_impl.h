@@ -101,13 +101,14 @@ struct Origin {
     long long in_base, out_base;
 };
 
-// Row tiles of 256-point lines on the matrix cores: MFMA group P of a stage is line P in
-// both radix-16 stages (16 slots of a line per group) and P = wave + u NW, so every line's
-// LDS exchange, and the next tile's overwrite of it, stays inside one wave. LDS operations of
-// a wave execute in order, so such a tile needs no workgroup barrier between its stages
-// or before the next tile, only a point the compiler does not move LDS accesses across.
+// Row tiles of 256-point lines: on the matrix cores MFMA group P of a stage is line P in
+// both radix-16 stages (16 slots of a line per group) and P = wave + u NW; in the fp32 path
+// thread i holds a column of line i / 16 in both stages. Either way every line's LDS
+// exchange, and the next tile's overwrite of it, stays inside one wave. LDS operations of a
+// wave execute in order, so such a tile needs no workgroup barrier between its stages or
+// before the next tile, only a point the compiler does not move LDS accesses across.
 template <int PREC, int R, int KIND>
-constexpr bool wave_lines() { return PREC != PREC_FP32 && KIND == KIND_ROWS && R == 256; }
+constexpr bool wave_lines() { return KIND == KIND_ROWS && R == 256; }
 
 template <int PREC, int R, int KIND>
 __device__ __forceinline__ void tile_sync()
@@ -612,7 +613,7 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
     } else {
 #pragma unroll
         for (int j = 0; j < 16; ++j) v[j] = lds[G::phys(t, col + j * R16)];
-        if constexpr (!LAST) __syncthreads();
+        if constexpr (!LAST) tile_sync<PREC_FP32, R, KIND>();
     }
     dft16v(v, a.sign, a.w16);
     if constexpr (!LAST) {
@@ -620,7 +621,7 @@ __device__ __forceinline__ void vstage16(float2 *lds, const TileArgs &a, const O
 #pragma unroll
         for (int tq = 0; tq < 16; ++tq)
             lds[G::phys(t, q + s * (16 * p + tq))] = cmulf(v[tq], stw[p * tq * s]);
-        __syncthreads();
+        tile_sync<PREC_FP32, R, KIND>();
     } else {
         wait_prefetch();
         if (t >= o.nv) return;
