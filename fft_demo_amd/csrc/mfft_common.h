@@ -128,6 +128,13 @@ struct TileArgs {
     unsigned in_span, out_span;
 };
 
+// Threads of a streaming launch with the fused filter product (TileArgs::in_mul): the lines
+// of a transform in whole groups of 64, times the launch's transforms.
+__host__ __device__ inline long long in_mul_threads(long long lines, long long transforms)
+{
+    return (lines + 63) / 64 * 64 * transforms;
+}
+
 // DFT16 operands for the A side of v_mfma_f32_16x16x32_{f16,bf16}: per lane 8 values of
 // [Fr | -s*Fi] (real output rows) and [s*Fi | Fr] (imag output rows), scaled by 16.
 struct alignas(16) DftOps {

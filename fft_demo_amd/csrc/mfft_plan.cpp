@@ -1024,6 +1024,7 @@ static int exec_chunk(mfft_plan_s *p, const char *in, char *out, long long nb, h
         }
         if (ps.small) {
             long long nl = (long long)a.c0 * a.c1 * a.c2;
+            if (a.in_mul) nl = in_mul_threads((long long)a.c0 * a.c1, a.c2);
             // largest byte offset a line's point reaches, per side (strides are >= 0)
             auto span = [&](long long s0, long long s1, long long s2, long long sp, int fmt) -> long long {
                 if (s0 < 0 || s1 < 0 || s2 < 0 || sp < 0) return -1;

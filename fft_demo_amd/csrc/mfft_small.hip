@@ -226,21 +226,28 @@ template <int R, int INF, int OUTF>
 __global__ void __launch_bounds__(256) small_cols(TileArgs a)
 {
     const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long nl = (long long)a.c0 * a.c1 * a.c2;
-    if (id >= nl) return;
+    const long long nlines = (long long)a.c0 * a.c1;  // lines per transform
+    const long long nl = nlines * a.c2;
     // line index -> (l0, l1, l2); 32-bit divisions whenever the line count allows
     int l0, l1;
     long long l2;
-    if (a.in_mul && nl <= 0xffffffffLL) {
-        // conv filter product: l2 (the transforms of the launch) varies fastest, so the lanes
-        // that multiply by one word of the filter spectrum sit in one wave and the word is
-        // fetched once per wave rather than once per transform
+    if (a.in_mul && in_mul_threads(nlines, a.c2) <= 0xffffffffLL) {
+        // conv filter product: 64 consecutive lines of one transform per wave, the same lines
+        // of the launch's other transforms (l2) in the waves that follow (the launch covers
+        // the lines in whole groups of 64). A wave then stores 64 consecutive words, and the
+        // filter-spectrum words of its lines, read by the c2 waves of a group close in time,
+        // come from memory once per launch.
+        if (id >= in_mul_threads(nlines, a.c2)) return;
         const unsigned u = (unsigned)id, c2 = (unsigned)a.c2, c0 = (unsigned)a.c0;
-        const unsigned r = u / c2;
-        l2 = u - r * c2;
-        const unsigned q = r / c0;
-        l0 = (int)(r - q * c0);
+        const unsigned grp = u / (64u * c2), rem = u - grp * (64u * c2);
+        const unsigned line = grp * 64u + (rem & 63u);
+        if (line >= (unsigned)nlines) return;
+        l2 = rem >> 6;
+        const unsigned q = line / c0;
+        l0 = (int)(line - q * c0);
         l1 = (int)q;
+    } else if (id >= nl) {
+        return;
     } else if (nl <= 0xffffffffLL) {
         const unsigned u = (unsigned)id, c0 = (unsigned)a.c0, c1 = (unsigned)a.c1;
         const unsigned r = u / c0;
