@@ -36,6 +36,19 @@ def test_n256_batch4096(dev, prec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["split_fp16", "fp16", "bf16", "fp32"])
+@pytest.mark.parametrize("batch", [1, 13, 4093, 20000])
+def test_n256_ragged_tiles(dev, prec, batch):
+    """256-point row tiles hold 16 lines and order their stages per wave (each line stays in
+    one wave); batches that leave a partial last tile, one tile, and more tiles than the
+    persistent grid (several tiles per workgroup) against the oracle."""
+    y, ref, plan = _run(dev, 256, batch, prec, seed=batch % 97)
+    l2, mx = rel_errors(y, ref)
+    print(f"N=256 b={batch} {prec}: rel_l2={l2:.3e} rel_max={mx:.3e}")
+    assert l2 <= TOL[prec][0] and mx <= TOL[prec][1]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n", [16, 32, 64, 128, 512, 1024, 2048, 4096, 8192, 65536])
 def test_sizes_split(dev, n):
     batch = max(1, min(64, (1 << 20) // n))
