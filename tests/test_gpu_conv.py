@@ -204,12 +204,15 @@ def test_conv_same_fp16_spectra(dev, B, C, F, h, w, fh, fw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("B,h,w,fh,fw,pad", [(3, 120, 136, 9, 7, None), (2, 100, 90, 5, 5, (128, 128))])
+@pytest.mark.parametrize("B,h,w,fh,fw,pad", [(3, 120, 136, 9, 7, None), (2, 100, 90, 5, 5, (128, 128)),
+                                              (5, 80, 130, 7, 7, (90, 144))])
 def test_conv_same_fp32_spectra_fp16_boundaries(dev, B, h, w, fh, fw, pad):
     """fp32 image spectra with fp16 pass boundaries inside each transform (mid_fp16), three
     filters, each selected per call; padded 128 x 144 (odd part 9: image load and filter
-    product fused into the first passes) and 128 x 128 (no fused pass: the packed images and
-    the pointwise product go through the transforms out of place). Tolerance as for
+    product fused into the first passes), 128 x 128 (no fused pass: the packed images and
+    the pointwise product go through the transforms out of place) and 90 x 144 (the fused
+    filter product's radix-9 pass has 16 x 90 = 1440 lines per transform, not a whole number
+    of the 64-line groups its waves take, and three pairs per launch). Tolerance as for
     spec_fp16: rel-L2 <= 1e-3 and rel-max <= 1e-2 of each output map. A call for all filters
     at once is refused (one filter per call)."""
     import torch
